@@ -1,0 +1,178 @@
+/*
+ * kplex.h -- C ABI of the MI355X-native maximal k-plex enumeration library
+ * (libkplex.so).  Plain C types only; no HIP or torch types cross it.
+ *
+ * The reference (CudaPlex) has no library boundary: it is one executable
+ * whose de-facto interface is
+ *   - the binary CSR file          inc/graph.h:37-65
+ *   - the CLI  main <g> -k -q -t -tr   main.cu:11-62
+ *   - the parameter globals k, lb(=q), thres, truss, bd   inc/kPlexEnum.h:20-24
+ *   - the entry void decomposableSearch(const graph<int>&)  inc/host_funcs.h:722
+ *   - its stdout lines              inc/host_funcs.h:1149-1157, inc/graph.h:47
+ * Each entry point below names the piece of that interface it replaces.
+ *
+ * Conventions: every call returns 0 (KP_OK) or a negative KP_E* code.  The
+ * caller owns its inputs; the library owns what it returns through out
+ * structs and frees it with the matching *_free call.  Handles are not
+ * shared between threads without external locking.
+ */
+#ifndef KPLEX_H
+#define KPLEX_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------- */
+#define KP_OK          0
+#define KP_ENOENT     (-2)    /* file not found (reference: utils::open_file exits, inc/utils.h:24-33) */
+#define KP_EIO        (-5)    /* short read / write */
+#define KP_ENOMEM     (-12)   /* host or device allocation failed */
+#define KP_ENODEV     (-19)   /* no HIP device / library built without one */
+#define KP_EINVAL     (-22)   /* bad argument (k < 1, q < 2k, malformed graph) */
+#define KP_ERANGE     (-34)   /* a root block exceeds KP_MAX_BLOCK vertices */
+#define KP_EOVERFLOW  (-75)   /* an internal capacity (queue, stack, list buffer) was exceeded */
+#define KP_ETIMEDOUT  (-110)  /* the time limit in kp_params was reached; results are partial */
+#define KP_EBUSY      (-16)   /* server: admission refused (memory / slots) */
+#define KP_EHIP       (-1000) /* HIP runtime error; kp_last_error() has the text */
+
+#define KP_MAX_BLOCK  1024    /* largest root block (reference MAX_BLK_SIZE, inc/common.h:8) */
+
+/* ---- graph (inc/graph.h) -------------------------------------------- */
+typedef struct kp_graph {
+    uint32_t  n;     /* vertices */
+    uint64_t  m;     /* directed adjacency entries = 2 x undirected edges */
+    uint64_t *off;   /* n+1 offsets */
+    uint32_t *nbr;   /* m neighbour ids, each list sorted ascending */
+    uint32_t *deg;   /* n degrees */
+} kp_graph;
+
+/* Replaces graph<intT>::graph(const char*) (inc/graph.h:37-65): reads
+ * u32 n, u32 m, u32 deg[n], u32 nbrs[m].  Unlike the reference it fails
+ * with KP_ENOENT/KP_EIO/KP_EINVAL instead of exiting or reading past the end. */
+int  kp_graph_load_bin(const char *path, kp_graph *out);
+/* Writes the same format (the write order of inc/graphIO.h:19-25). */
+int  kp_graph_save_bin(const kp_graph *g, const char *path);
+/* Copies a CSR given as degrees + concatenated neighbour lists. */
+int  kp_graph_from_csr(uint32_t n, const uint32_t *deg, const uint32_t *nbr, kp_graph *out);
+/* Checks sortedness, symmetry and absence of self loops / duplicates
+ * (the reference assumes all of them: its lookups are binary searches). */
+int  kp_graph_validate(const kp_graph *g);
+void kp_graph_free(kp_graph *g);
+
+/* ---- parameters (inc/kPlexEnum.h:20-24; main.cu:31-53) -------------- */
+typedef struct kp_params {
+    int      k;            /* -k : at most k-1 non-neighbours per vertex inside the plex */
+    int      q;            /* -q : minimum plex size (reference global lb) */
+    float    thres;        /* -t : pause threshold, fraction of the device task-queue capacity */
+    int      truss;        /* -tr: 1 = (q-2k)-truss prefilter before the core */
+    uint32_t flags;        /* KP_FLAG_* */
+    double   time_limit_s; /* 0 = none; else KP_ETIMEDOUT after this many seconds of search */
+    uint64_t queue_capacity; /* device task-queue slots; 0 = library default */
+} kp_params;
+
+#define KP_FLAG_LIST      1u  /* return every maximal k-plex (original ids, sorted) */
+#define KP_FLAG_VALIDATE  2u  /* run kp_graph_validate first */
+#define KP_FLAG_VERBOSE   4u  /* print the reference's progress lines to stdout */
+
+/* ---- result (inc/host_funcs.h:1144-1157) ---------------------------- */
+typedef struct kp_result {
+    uint64_t count;        /* maximal k-plexes of size >= q ("Maximal k-Plexes", 64-bit) */
+    uint64_t tasks;        /* Branch tasks produced ("Total tasks generated") */
+    uint64_t valid_roots;  /* roots whose block has >= q vertices ("Total Valid Blocks") */
+    uint64_t hash_a, hash_b; /* order-independent 128-bit multiset hash, see below */
+    uint32_t pn;           /* vertices after preprocessing ("pn = ") */
+    uint64_t pm;           /* directed entries after preprocessing */
+    uint32_t max_n, max_l; /* largest root block / left set */
+    uint64_t alg_bytes;    /* sum_r [4 W_r + ceil(n_r^2/8) + n_r^2] + sum_tasks 2 (5 n + 16) */
+    uint64_t adj_bytes;    /* sum_r 4 W_r (CSR bytes read to build the root subgraphs) */
+    uint64_t search_alg_bytes; /* sum_tasks 2 (5 n + 16): the search kernels' share of alg_bytes */
+    double   preprocess_ms;/* host core/truss/degeneracy */
+    double   upload_ms;    /* host -> HBM copies */
+    double   decompose_ms; /* GPU root decomposition + subgraph build */
+    double   search_ms;    /* GPU persistent k-search + Branch */
+    double   wall_ms;      /* preprocess + decompose + search (reference "Total Time Elapsed") */
+    uint64_t spilled;      /* tasks spilled to pinned host memory */
+    uint64_t pauses;       /* pause/resume cycles */
+    uint64_t nlisted;      /* KP_FLAG_LIST: plex i is list_ids[list_off[i] .. list_off[i+1]) */
+    uint64_t *list_off;
+    uint32_t *list_ids;
+} kp_result;
+/*
+ * Hash: for each maximal k-plex S of original vertex ids
+ *   a(S) = mix64( sum_{v in S} mix64(v ^ 0x243F6A8885A308D3) + |S| * 0x13198A2E03707344 )
+ *   b(S) = mix64( sum_{v in S} mix64(v ^ 0xA4093822299F31D0) + |S| * 0x082EFA98EC4E6C89 )
+ * hash_a = sum_S a(S), hash_b = sum_S b(S) (mod 2^64), mix64 = SplitMix64
+ * finalizer (Steele, Lea & Flood 2014).  Independent of enumeration order.
+ */
+
+/* Replaces decomposableSearch(const graph<int>&) (inc/host_funcs.h:722-1162)
+ * with the globals k, lb, thres, truss passed in *p: preprocessing on the
+ * host, enumeration on the current HIP device. */
+int  kp_enumerate(const kp_graph *g, const kp_params *p, kp_result *r);
+void kp_result_free(kp_result *r);
+
+/* Host preprocessing only (no device needed): the (q-k)-core after the
+ * optional truss (inc/host_funcs.h:749-823), its new->original id map and the
+ * degeneracy order (inc/host_funcs.h:829-851).  out receives the preprocessed
+ * CSR; orig and dseq must have room for g->n entries. */
+int  kp_preprocess(const kp_graph *g, int k, int q, int truss, kp_graph *out,
+                   uint32_t *orig, int32_t *dseq);
+
+/* ---- prepared (HBM-resident) graphs --------------------------------- */
+/* Split of kp_enumerate: kp_prepare runs the host preprocessing for (k, q,
+ * truss) and uploads the preprocessed CSR + degeneracy order to HBM;
+ * kp_run then enumerates with every input already resident.  No reference
+ * counterpart: the reference re-runs the whole pipeline per invocation. */
+typedef struct kp_prepared kp_prepared;
+int  kp_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out);
+int  kp_run(kp_prepared *h, kp_result *r);
+uint64_t kp_prepared_bytes(const kp_prepared *h);
+void kp_prepared_free(kp_prepared *h);
+
+/* ---- batched: many graphs in one persistent launch ------------------ */
+/* No reference counterpart (the reference handles one graph per process).
+ * p may hold one kp_params per graph (stride = sizeof(kp_params)). */
+int  kp_enumerate_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r);
+
+/* ---- served: continuous batching of (graph, k, q) queries ----------- */
+typedef struct kp_server kp_server;
+typedef struct kp_server_config {
+    size_t   max_inflight;      /* queries on the device at once (0 = default) */
+    uint64_t mem_budget_bytes;  /* admission budget for projected stack/queue bytes (0 = default) */
+} kp_server_config;
+int  kp_server_create(const kp_server_config *cfg, kp_server **out);
+/* Registers a graph; it stays resident in HBM until the server is destroyed. */
+int  kp_server_add_graph(kp_server *s, const kp_graph *g, uint32_t *graph_id);
+/* Queues a query; returns its ticket.  Admission happens in kp_server_poll. */
+int  kp_server_submit(kp_server *s, uint32_t graph_id, const kp_params *p, uint64_t *ticket);
+/* Drives the server; returns completed queries (up to cap) in tickets/results. */
+int  kp_server_poll(kp_server *s, uint64_t *tickets, kp_result *results, size_t cap, size_t *ndone);
+int  kp_server_destroy(kp_server *s);
+
+/* ---- misc ------------------------------------------------------------ */
+const char *kp_strerror(int code);
+const char *kp_last_error(void);
+/* Fills buf with "<device name>, <CUs> CUs, <HBM GiB> GiB" of the current device. */
+int  kp_device_info(char *buf, size_t len);
+
+/* ---- synthetic inputs (data layer; no reference counterpart) ---------- */
+/* G(n, p) plus `groups` disjoint planted groups of `gsize` vertices, each a
+ * clique from which up to `max_removed` edges per vertex are removed. */
+int  kp_gen_er_planted(uint32_t n, double p, uint32_t groups, uint32_t gsize,
+                       uint32_t max_removed, uint64_t seed, kp_graph *out);
+/* Chung-Lu graph with power-law expected degrees (exponent gamma, largest
+ * expected degree max_deg, about m_target undirected edges) plus `ncomm`
+ * disjoint communities of size U[cmin, cmax] at intra-density `dens`. */
+int  kp_gen_chung_lu(uint32_t n, double gamma, uint64_t m_target, double max_deg,
+                     uint32_t ncomm, uint32_t cmin, uint32_t cmax, double dens,
+                     uint64_t seed, kp_graph *out);
+/* Applies a seeded random permutation to the vertex ids. */
+int  kp_graph_relabel(const kp_graph *g, uint64_t seed, kp_graph *out, uint32_t *perm_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

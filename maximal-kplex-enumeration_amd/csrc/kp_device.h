@@ -1,0 +1,16 @@
+// kp_device.h -- host-callable entry points of the HIP part of libkplex.
+#pragma once
+#include "kp_internal.h"
+
+struct kp_prepared;
+
+namespace kp {
+
+int  dev_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out);
+int  dev_run(kp_prepared *h, kp_result *r);
+void dev_free(kp_prepared *h);
+uint64_t dev_bytes(const kp_prepared *h);
+int  dev_info(char *buf, size_t len);
+const char *last_error();
+
+} // namespace kp

@@ -1,0 +1,51 @@
+// kp_internal.h -- shared declarations of the host side of libkplex.
+#pragma once
+#include "../../include/kplex.h"
+
+#include <cstdint>
+#include <vector>
+
+namespace kp {
+
+// SplitMix64 stream (Steele, Lea & Flood 2014).
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    static uint64_t mix(uint64_t z)
+    {
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        return z ^ (z >> 31);
+    }
+    uint64_t next() { return mix(s += 0x9E3779B97F4A7C15ULL); }
+    double uniform() { return double(next() >> 11) * (1.0 / 9007199254740992.0); }
+    uint32_t below(uint32_t n) { return n ? uint32_t((unsigned __int128)next() * n >> 64) : 0; }
+};
+
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+int graph_alloc(kp_graph *g, uint32_t n, uint64_t m);
+int graph_from_edges(uint32_t n, std::vector<uint64_t> &keys, kp_graph *out);
+
+// Host preprocessing result: the (q-k)-core (after the optional truss and
+// second core), its new->original id map and the degeneracy order.
+struct Prep {
+    kp_graph g{};
+    std::vector<uint32_t> orig;
+    std::vector<int32_t> dseq, dpos;
+    Prep() = default;
+    Prep(const Prep &) = delete;
+    Prep &operator=(const Prep &) = delete;
+    ~Prep() { kp_graph_free(&g); }
+};
+int preprocess(const kp_graph *g, int k, int q, int truss, Prep *out);
+
+double now_ms();
+
+// hash constants of the result hash (documented in kplex.h)
+constexpr uint64_t HASH_SA = 0x243F6A8885A308D3ULL;
+constexpr uint64_t HASH_LA = 0x13198A2E03707344ULL;
+constexpr uint64_t HASH_SB = 0xA4093822299F31D0ULL;
+constexpr uint64_t HASH_LB = 0x082EFA98EC4E6C89ULL;
+
+} // namespace kp

@@ -1,0 +1,320 @@
+"""ctypes binding of libkplex.so (the C ABI in include/kplex.h).
+
+This is the Python face of the drop-in boundary: plain pointers and sizes go
+through ctypes, no torch types.  The library itself is C++/HIP; nothing here
+computes anything on the enumeration path.
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_size_t, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "libkplex.so")
+CLI_PATH = os.path.join(HERE, "build", "PlexEnum")
+
+KP_OK = 0
+KP_ENOENT = -2
+KP_EIO = -5
+KP_ENOMEM = -12
+KP_ENODEV = -19
+KP_EINVAL = -22
+KP_ERANGE = -34
+KP_EOVERFLOW = -75
+KP_ETIMEDOUT = -110
+KP_EBUSY = -16
+KP_EHIP = -1000
+
+KP_FLAG_LIST = 1
+KP_FLAG_VALIDATE = 2
+KP_FLAG_VERBOSE = 4
+
+
+class KpGraph(ctypes.Structure):
+    _fields_ = [("n", c_uint32), ("m", c_uint64), ("off", POINTER(c_uint64)),
+                ("nbr", POINTER(c_uint32)), ("deg", POINTER(c_uint32))]
+
+
+class KpParams(ctypes.Structure):
+    _fields_ = [("k", c_int), ("q", c_int), ("thres", c_float), ("truss", c_int), ("flags", c_uint32),
+                ("time_limit_s", c_double), ("queue_capacity", c_uint64)]
+
+
+class KpResult(ctypes.Structure):
+    _fields_ = [("count", c_uint64), ("tasks", c_uint64), ("valid_roots", c_uint64),
+                ("hash_a", c_uint64), ("hash_b", c_uint64), ("pn", c_uint32), ("pm", c_uint64),
+                ("max_n", c_uint32), ("max_l", c_uint32), ("alg_bytes", c_uint64), ("adj_bytes", c_uint64), ("search_alg_bytes", c_uint64),
+                ("preprocess_ms", c_double), ("upload_ms", c_double), ("decompose_ms", c_double),
+                ("search_ms", c_double), ("wall_ms", c_double), ("spilled", c_uint64), ("pauses", c_uint64),
+                ("nlisted", c_uint64), ("list_off", POINTER(c_uint64)), ("list_ids", POINTER(c_uint32))]
+
+
+class KpServerConfig(ctypes.Structure):
+    _fields_ = [("max_inflight", c_size_t), ("mem_budget_bytes", c_uint64)]
+
+
+# every entry point include/kplex.h declares: name -> (restype, argtypes)
+SIGNATURES = {
+    "kp_graph_load_bin": (c_int, [c_char_p, POINTER(KpGraph)]),
+    "kp_graph_save_bin": (c_int, [POINTER(KpGraph), c_char_p]),
+    "kp_graph_from_csr": (c_int, [c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(KpGraph)]),
+    "kp_graph_validate": (c_int, [POINTER(KpGraph)]),
+    "kp_graph_free": (None, [POINTER(KpGraph)]),
+    "kp_enumerate": (c_int, [POINTER(KpGraph), POINTER(KpParams), POINTER(KpResult)]),
+    "kp_result_free": (None, [POINTER(KpResult)]),
+    "kp_prepare": (c_int, [POINTER(KpGraph), POINTER(KpParams), POINTER(c_void_p)]),
+    "kp_run": (c_int, [c_void_p, POINTER(KpResult)]),
+    "kp_prepared_bytes": (c_uint64, [c_void_p]),
+    "kp_prepared_free": (None, [c_void_p]),
+    "kp_enumerate_batch": (c_int, [POINTER(POINTER(KpGraph)), POINTER(KpParams), c_size_t, POINTER(KpResult)]),
+    "kp_server_create": (c_int, [POINTER(KpServerConfig), POINTER(c_void_p)]),
+    "kp_server_add_graph": (c_int, [c_void_p, POINTER(KpGraph), POINTER(c_uint32)]),
+    "kp_server_submit": (c_int, [c_void_p, c_uint32, POINTER(KpParams), POINTER(c_uint64)]),
+    "kp_server_poll": (c_int, [c_void_p, POINTER(c_uint64), POINTER(KpResult), c_size_t, POINTER(c_size_t)]),
+    "kp_server_destroy": (c_int, [c_void_p]),
+    "kp_strerror": (c_char_p, [c_int]),
+    "kp_last_error": (c_char_p, []),
+    "kp_device_info": (c_int, [c_char_p, c_size_t]),
+    "kp_gen_er_planted": (c_int, [c_uint32, c_double, c_uint32, c_uint32, c_uint32, c_uint64, POINTER(KpGraph)]),
+    "kp_gen_chung_lu": (c_int, [c_uint32, c_double, c_uint64, c_double, c_uint32, c_uint32, c_uint32, c_double,
+                                c_uint64, POINTER(KpGraph)]),
+    "kp_graph_relabel": (c_int, [POINTER(KpGraph), c_uint64, POINTER(KpGraph), POINTER(c_uint32)]),
+    "kp_preprocess": (c_int, [POINTER(KpGraph), c_int, c_int, c_int, POINTER(KpGraph), POINTER(c_uint32),
+                              POINTER(ctypes.c_int32)]),
+}
+
+_lib = None
+
+
+class KplexError(RuntimeError):
+    def __init__(self, code, where):
+        lib = load()
+        msg = lib.kp_strerror(code).decode()
+        detail = lib.kp_last_error().decode()
+        super().__init__(f"{where}: {msg} ({code}): {detail}")
+        self.code = code
+
+
+def load(path=None):
+    """Loads libkplex.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} is missing: build it with `make -C maximal-kplex-enumeration_amd`")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc, where):
+    if rc != KP_OK:
+        raise KplexError(rc, where)
+
+
+def device_info():
+    lib = load()
+    buf = ctypes.create_string_buffer(256)
+    _check(lib.kp_device_info(buf, 256), "kp_device_info")
+    return buf.value.decode()
+
+
+class Graph:
+    """Owns a kp_graph allocated by the library."""
+
+    def __init__(self):
+        self.g = KpGraph()
+
+    def __del__(self):
+        try:
+            load().kp_graph_free(ctypes.byref(self.g))
+        except Exception:
+            pass
+
+    @property
+    def n(self):
+        return self.g.n
+
+    @property
+    def m(self):
+        return self.g.m
+
+    @classmethod
+    def load_bin(cls, path):
+        self = cls()
+        _check(load().kp_graph_load_bin(str(path).encode(), ctypes.byref(self.g)), f"load {path}")
+        return self
+
+    def save_bin(self, path):
+        _check(load().kp_graph_save_bin(ctypes.byref(self.g), str(path).encode()), f"save {path}")
+
+    @classmethod
+    def from_csr(cls, deg, nbr):
+        deg = np.ascontiguousarray(deg, dtype=np.uint32)
+        nbr = np.ascontiguousarray(nbr, dtype=np.uint32)
+        self = cls()
+        _check(load().kp_graph_from_csr(len(deg), deg.ctypes.data_as(POINTER(c_uint32)),
+                                        nbr.ctypes.data_as(POINTER(c_uint32)), ctypes.byref(self.g)),
+               "kp_graph_from_csr")
+        return self
+
+    @classmethod
+    def from_edges(cls, n, edges):
+        """Simple undirected graph from an iterable of (u, v) pairs."""
+        adj = [set() for _ in range(n)]
+        for u, v in edges:
+            if u != v:
+                adj[u].add(v)
+                adj[v].add(u)
+        deg = np.array([len(a) for a in adj], dtype=np.uint32)
+        nbr = np.array([x for a in adj for x in sorted(a)], dtype=np.uint32)
+        return cls.from_csr(deg, nbr)
+
+    def csr(self):
+        n, m = self.g.n, self.g.m
+        deg = np.ctypeslib.as_array(self.g.deg, shape=(max(n, 1),))[:n].copy()
+        nbr = np.ctypeslib.as_array(self.g.nbr, shape=(max(m, 1),))[:m].copy()
+        return deg, nbr
+
+    def validate(self):
+        return load().kp_graph_validate(ctypes.byref(self.g))
+
+    def relabel(self, seed):
+        out = Graph()
+        perm = np.zeros(max(self.n, 1), dtype=np.uint32)
+        _check(load().kp_graph_relabel(ctypes.byref(self.g), seed, ctypes.byref(out.g),
+                                       perm.ctypes.data_as(POINTER(c_uint32))), "kp_graph_relabel")
+        return out, perm[:self.n]
+
+    @classmethod
+    def er_planted(cls, n, p, groups, gsize, max_removed, seed):
+        self = cls()
+        _check(load().kp_gen_er_planted(n, p, groups, gsize, max_removed, seed, ctypes.byref(self.g)),
+               "kp_gen_er_planted")
+        return self
+
+    @classmethod
+    def chung_lu(cls, n, gamma, m_target, max_deg, ncomm, cmin, cmax, dens, seed):
+        self = cls()
+        _check(load().kp_gen_chung_lu(n, gamma, m_target, max_deg, ncomm, cmin, cmax, dens, seed,
+                                      ctypes.byref(self.g)), "kp_gen_chung_lu")
+        return self
+
+
+def preprocess(graph, k, q, truss=0):
+    """Host preprocessing only: returns (peeled CSR deg, nbr, orig ids, degeneracy order)."""
+    lib = load()
+    out = Graph()
+    n = max(graph.n, 1)
+    orig = np.zeros(n, dtype=np.uint32)
+    dseq = np.zeros(n, dtype=np.int32)
+    _check(lib.kp_preprocess(ctypes.byref(graph.g), k, q, truss, ctypes.byref(out.g),
+                             orig.ctypes.data_as(POINTER(c_uint32)), dseq.ctypes.data_as(POINTER(ctypes.c_int32))),
+           "kp_preprocess")
+    deg, nbr = out.csr()
+    return deg, nbr, orig[:out.n], dseq[:out.n]
+
+
+def _result_dict(r, listed):
+    d = {f: getattr(r, f) for f, _ in KpResult._fields_ if f not in ("list_off", "list_ids")}
+    d["hash"] = f"{r.hash_b:016x}{r.hash_a:016x}"
+    if listed:
+        off = np.ctypeslib.as_array(r.list_off, shape=(r.nlisted + 1,)).copy() if r.nlisted else np.zeros(1, np.uint64)
+        ids = np.ctypeslib.as_array(r.list_ids, shape=(int(off[-1]),)).copy() if off[-1] else np.zeros(0, np.uint32)
+        d["plexes"] = [tuple(int(x) for x in ids[off[i]:off[i + 1]]) for i in range(int(r.nlisted))]
+    return d
+
+
+def make_params(k, q, thres=0.25, truss=0, flags=0, time_limit_s=0.0, queue_capacity=0):
+    return KpParams(k=k, q=q, thres=thres, truss=truss, flags=flags, time_limit_s=time_limit_s,
+                    queue_capacity=queue_capacity)
+
+
+def enumerate_plexes(graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0, validate=False):
+    """kp_enumerate: returns a dict with count, hash, tasks, timings (and plexes when listing)."""
+    lib = load()
+    flags = (KP_FLAG_LIST if listing else 0) | (KP_FLAG_VALIDATE if validate else 0)
+    p = make_params(k, q, thres, truss, flags, time_limit_s)
+    r = KpResult()
+    _check(lib.kp_enumerate(ctypes.byref(graph.g), ctypes.byref(p), ctypes.byref(r)), "kp_enumerate")
+    try:
+        return _result_dict(r, listing)
+    finally:
+        lib.kp_result_free(ctypes.byref(r))
+
+
+class Prepared:
+    """kp_prepare / kp_run: the graph preprocessed and resident in HBM."""
+
+    def __init__(self, graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0):
+        self.lib = load()
+        self.h = c_void_p()
+        self.listing = listing
+        p = make_params(k, q, thres, truss, KP_FLAG_LIST if listing else 0, time_limit_s)
+        _check(self.lib.kp_prepare(ctypes.byref(graph.g), ctypes.byref(p), ctypes.byref(self.h)), "kp_prepare")
+
+    def run(self):
+        r = KpResult()
+        _check(self.lib.kp_run(self.h, ctypes.byref(r)), "kp_run")
+        try:
+            return _result_dict(r, self.listing)
+        finally:
+            self.lib.kp_result_free(ctypes.byref(r))
+
+    def bytes(self):
+        return self.lib.kp_prepared_bytes(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.kp_prepared_free(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- workloads
+# The five synthetic workloads (seeds 1..5).  Each returns (graph, k, q, thres, truss).
+def workload_er2000():
+    """G(2000, 0.01) + 20 planted groups of 40 (clique minus <= 2 edges per vertex); k=3 q=20."""
+    return Graph.er_planted(2000, 0.01, 20, 40, 2, 1), 3, 20, 0.25, 0
+
+def workload_cl75k():
+    """Chung-Lu n=75,000 (exponent 2.1, max expected degree 3,000, ~508k edges)
+    + 300 communities of 35-60 at density 0.9; k=4 q=30."""
+    return Graph.chung_lu(75000, 2.1, 508000, 3000.0, 300, 35, 60, 0.9, 2), 4, 30, 0.25, 0
+
+
+def workload_cl320k():
+    """Chung-Lu n=320,000 (exponent 2.0, max expected degree 5,000, ~1.05M edges)
+    + 1,000 communities of 30-80 at density 0.85; k=5 q=30, truss on."""
+    return Graph.chung_lu(320000, 2.0, 1050000, 5000.0, 1000, 30, 80, 0.85, 5), 5, 30, 0.25, 1
+
+
+def batch_graph_specs(count=4096, nmax=20000, seed=3, kset=(2, 3, 4), qfun=lambda k: 3 * k + 8):
+    """Specs of the many-graph workload: n log-uniform in [200, nmax], Chung-Lu
+    exponent 2.3 with average degree 10, 1-20 communities of 25-50 at density 0.9."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        n = int(np.floor(np.exp(rng.uniform(np.log(200), np.log(nmax)))))
+        k = int(rng.choice(kset))
+        ncomm = int(rng.integers(1, 21))
+        specs.append(dict(n=n, gamma=2.3, m_target=5 * n, max_deg=float(max(8.0, 4.0 * np.sqrt(n))),
+                          ncomm=ncomm, cmin=25, cmax=50, dens=0.9, seed=seed * 1_000_003 + i, k=k, q=qfun(k)))
+    return specs
+
+
+def graph_from_spec(s):
+    return Graph.chung_lu(s["n"], s["gamma"], s["m_target"], s["max_deg"], s["ncomm"], s["cmin"], s["cmax"],
+                          s["dens"], s["seed"])

@@ -1,0 +1,173 @@
+"""GPU parity: libkplex's HIP path against the oracle, brute force, closed
+forms, the README known answers and the recorded golden fixtures."""
+import json
+import math
+import os
+import sys
+
+import pytest
+
+import kplex
+import kpo
+from conftest import GOLDEN
+from kp_testutil import brute_force, csr, disjoint_cliques, multipartite, random_graph
+
+pytestmark = pytest.mark.gpu
+
+LIMIT = 90.0   # seconds of search per call; every kernel exits at this bound
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def gpu(graph, k, q, truss=0, listing=False, name=""):
+    say(f"{name} n={graph.n} m={graph.m} k={k} q={q} truss={truss}")
+    return kplex.enumerate_plexes(graph, k, q, truss=truss, listing=listing, time_limit_s=LIMIT)
+
+
+def oracle(graph, k, q, truss=0, listing=False):
+    deg, nbr = graph.csr()
+    return kpo.enumerate_plexes(kpo.OGraph.from_csr(deg, nbr), k, q, truss, listing=listing)
+
+
+def same(a, b, keys=("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l")):
+    for x in keys:
+        assert a[x] == b[x], (x, a[x], b[x])
+
+
+@pytest.fixture(scope="module")
+def gold():
+    with open(os.path.join(GOLDEN, "oracle_results.json")) as f:
+        return json.load(f)
+
+
+def test_device_visible():
+    say("device info")
+    info = kplex.device_info()
+    assert "gfx9" in info
+
+
+@pytest.mark.parametrize("parts,t,k", [(3, 3, 2), (4, 3, 2), (2, 5, 3), (6, 3, 2), (5, 4, 3), (4, 4, 4)])
+def test_multipartite_closed_form(parts, t, k):
+    g = kplex.Graph.from_csr(*csr(multipartite(parts, t)))
+    r = gpu(g, k, parts * k, listing=True, name=f"K_{parts}x{t}")
+    assert r["count"] == math.comb(t, k) ** parts
+    same(r, oracle(g, k, parts * k))
+
+
+def test_disjoint_cliques():
+    g = kplex.Graph.from_csr(*csr(disjoint_cliques([5, 6, 3, 7, 70, 130])))
+    r = gpu(g, 2, 4, listing=True, name="cliques")
+    assert r["count"] == 5
+    same(r, oracle(g, 2, 4))
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_brute_force(seed):
+    n = 13 + seed % 3
+    k = 2 + seed % 2
+    q = 2 * k + seed % 3
+    adj = random_graph(n, 0.72, 1000 + seed)
+    g = kplex.Graph.from_csr(*csr(adj))
+    r = gpu(g, k, q, listing=True, name=f"brute{seed}")
+    want = brute_force(n, adj, k, q)
+    assert [tuple(p) for p in r["plexes"]] == want
+    assert r["hash"] == kpo.hash_plexes(want)
+
+
+@pytest.mark.parametrize("seed,k,q", [(1, 2, 5), (2, 3, 7), (3, 2, 6), (4, 3, 8), (5, 4, 9)])
+def test_random_vs_oracle(seed, k, q):
+    adj = random_graph(160, 0.28, 300 + seed)
+    g = kplex.Graph.from_csr(*csr(adj))
+    r = gpu(g, k, q, name=f"random{seed}")
+    o = oracle(g, k, q)
+    assert o["count"] > 0
+    same(r, o)
+
+
+@pytest.mark.parametrize("gsize,k,q", [(70, 2, 30), (140, 2, 60), (200, 3, 60), (330, 2, 120), (600, 2, 300)])
+def test_size_classes(gsize, k, q):
+    # groups larger than 64/128/192/256/512 vertices exercise every bitset width
+    g = kplex.Graph.er_planted(1500, 0.004, 2, gsize, 1, 40 + gsize)
+    r = gpu(g, k, q, listing=True, name=f"classes{gsize}")
+    o = oracle(g, k, q, listing=True)
+    assert r["max_n"] >= gsize - 5
+    same(r, o)
+    assert [tuple(p) for p in r["plexes"]] == o["plexes"]
+
+
+def test_er2000_set_diff(gold):
+    g, k, q, _, tr = kplex.workload_er2000()
+    r = gpu(g, k, q, tr, listing=True, name="er2000")
+    want = gold["er2000"]
+    assert [list(p) for p in r["plexes"]] == want["plexes"]
+    same(r, want, ("count", "hash", "tasks", "valid_roots", "pn", "pm"))
+
+
+@pytest.mark.parametrize("case", ["email-euall_k3_q12_tr0", "email-euall_k3_q12_tr1", "soc-epinions_k4_q30_tr0"])
+def test_readme_known_answers(datasets, gold, case):
+    name, ks, qs, trs = case.rsplit("_", 3)
+    k, q, tr = int(ks[1:]), int(qs[1:]), int(trs[2:])
+    g = kplex.Graph.load_bin(datasets[name])
+    r = gpu(g, k, q, tr, name=case)
+    readme = {"email-euall": 32639016, "soc-epinions": 13172906}[name]   # README.md:142,144
+    assert r["count"] == readme
+    same(r, gold[case], ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l"))
+
+
+def test_cl75k_against_golden(gold):
+    if "cl75k" not in gold:
+        pytest.skip("no recorded oracle result for the Chung-Lu 75k workload")
+    g, k, q, _, tr = kplex.workload_cl75k()
+    r = gpu(g, k, q, tr, name="cl75k")
+    same(r, gold["cl75k"], ("count", "hash", "tasks", "valid_roots", "pn", "pm"))
+
+
+def test_relabel_invariance():
+    g = kplex.Graph.er_planted(1200, 0.01, 6, 30, 3, 9)
+    a = gpu(g, 3, 14, listing=True, name="relabel-a")
+    h, perm = g.relabel(1234)
+    b = gpu(h, 3, 14, listing=True, name="relabel-b")
+    inv = {int(p): i for i, p in enumerate(perm)}
+    mapped = sorted(tuple(sorted(inv[v] for v in p)) for p in b["plexes"])
+    assert a["count"] == b["count"] > 0
+    assert mapped == [tuple(p) for p in a["plexes"]]
+
+
+def test_truss_invariance():
+    g = kplex.Graph.er_planted(1500, 0.012, 8, 28, 3, 21)
+    a = gpu(g, 3, 16, 0, name="truss0")
+    b = gpu(g, 3, 16, 1, name="truss1")
+    assert (a["count"], a["hash"]) == (b["count"], b["hash"])
+
+
+def test_prepared_runs_repeat_identically():
+    g = kplex.Graph.er_planted(1200, 0.01, 6, 30, 3, 9)
+    say("prepared x3")
+    p = kplex.Prepared(g, 3, 14, time_limit_s=LIMIT)
+    rs = [p.run() for _ in range(3)]
+    p.close()
+    for r in rs[1:]:
+        same(r, rs[0], ("count", "hash", "tasks", "valid_roots"))
+    same(rs[0], oracle(g, 3, 14))
+
+
+def test_invalid_params_refused():
+    g = kplex.Graph.from_csr(*csr(multipartite(3, 3)))
+    with pytest.raises(kplex.KplexError) as e:
+        kplex.enumerate_plexes(g, 3, 5)
+    assert e.value.code == kplex.KP_EINVAL
+
+
+def test_time_limit_is_reported():
+    # a limit of one microsecond: every wave sees the deadline and exits normally
+    g, k, q, _, tr = kplex.workload_er2000()
+    say("time-limit")
+    with pytest.raises(kplex.KplexError) as e:
+        kplex.enumerate_plexes(g, k, q, truss=tr, time_limit_s=1e-6)
+    assert e.value.code == kplex.KP_ETIMEDOUT
+    # the library is usable afterwards
+    r = gpu(g, k, q, tr, name="after-timeout")
+    assert r["count"] == 20

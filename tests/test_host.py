@@ -1,0 +1,146 @@
+"""Host-side checks of libkplex that need no GPU: the C ABI exports, the
+binary CSR loader, the generators and the host preprocessing (compared with
+the oracle's independent implementation)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import kplex
+import kpo
+from conftest import ROOT
+from kp_testutil import csr, random_graph
+
+
+def header_functions():
+    with open(os.path.join(ROOT, "include", "kplex.h")) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char|uint64_t)\s*\*?\s*(kp_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = kplex.load()
+    names = header_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(kplex.SIGNATURES), set(names) ^ set(kplex.SIGNATURES)
+
+
+def test_cli_usage_errors():
+    exe = kplex.CLI_PATH
+    r = subprocess.run([exe, "x.bin", "-k", "2"], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr
+    r = subprocess.run([exe, "x.bin", "-k", "2", "-q", "4", "-zz", "1"], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr
+    r = subprocess.run([exe, "/nonexistent.bin", "-k", "2", "-q", "4"], capture_output=True, text=True)
+    assert r.returncode == 1 and "file: /nonexistent.bin" in r.stdout
+
+
+def test_load_save_roundtrip(tmp_path, datasets):
+    g = kplex.Graph.load_bin(datasets["soc-epinions"])
+    assert (g.n, g.m) == (75879, 811480)
+    assert g.validate() == 0
+    p = tmp_path / "copy.bin"
+    g.save_bin(p)
+    with open(datasets["soc-epinions"], "rb") as a, open(p, "rb") as b:
+        assert a.read() == b.read()
+
+
+def test_loader_errors(tmp_path):
+    with pytest.raises(kplex.KplexError) as e:
+        kplex.Graph.load_bin(tmp_path / "missing.bin")
+    assert e.value.code == kplex.KP_ENOENT
+    bad = tmp_path / "short.bin"
+    np.array([3, 4, 1, 1], dtype=np.uint32).tofile(bad)   # truncated
+    with pytest.raises(kplex.KplexError) as e:
+        kplex.Graph.load_bin(bad)
+    assert e.value.code in (kplex.KP_EIO, kplex.KP_EINVAL)
+
+
+def test_validate_rejects_asymmetric_and_unsorted():
+    g = kplex.Graph.from_csr([1, 0], [1])          # 0->1 without 1->0
+    assert g.validate() == kplex.KP_EINVAL
+    g = kplex.Graph.from_csr([2, 1, 1], [2, 1, 0, 0])   # unsorted list of 0
+    assert g.validate() == kplex.KP_EINVAL
+    g = kplex.Graph.from_edges(4, [(0, 1), (1, 2), (2, 3)])
+    assert g.validate() == 0
+
+
+def test_generators_deterministic_and_simple():
+    a = kplex.Graph.er_planted(500, 0.02, 5, 20, 2, 7)
+    b = kplex.Graph.er_planted(500, 0.02, 5, 20, 2, 7)
+    assert a.validate() == 0
+    da, na = a.csr()
+    db, nb = b.csr()
+    assert np.array_equal(da, db) and np.array_equal(na, nb)
+    c = kplex.Graph.chung_lu(5000, 2.1, 20000, 300.0, 10, 20, 30, 0.9, 3)
+    assert c.validate() == 0
+    deg, _ = c.csr()
+    assert 0.8 * 2 * 20000 < deg.sum() < 1.6 * 2 * 20000 + 10 * 30 * 29
+
+
+def test_planted_groups_have_capped_missing_edges():
+    g = kplex.Graph.er_planted(300, 0.0, 3, 40, 2, 11)
+    deg, nbr = g.csr()
+    inside = deg[deg > 0]
+    assert len(inside) == 120
+    assert inside.min() >= 39 - 2
+
+
+def compare_prep(graph, k, q, truss):
+    deg, nbr = graph.csr()
+    a = kplex.preprocess(graph, k, q, truss)
+    b = kpo.preprocess(kpo.OGraph.from_csr(deg, nbr), k, q, truss)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    return len(a[0])
+
+
+@pytest.mark.parametrize("truss", [0, 1])
+def test_preprocess_matches_oracle_datasets(datasets, truss):
+    g = kplex.Graph.load_bin(datasets["email-euall"])
+    pn = compare_prep(g, 3, 12, truss)
+    assert pn == (3156 if not truss else 1086)
+    g = kplex.Graph.load_bin(datasets["soc-epinions"])
+    assert compare_prep(g, 4, 30, truss) == (4315 if not truss else compare_prep(g, 4, 30, truss))
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_preprocess_matches_oracle_random(seed):
+    adj = random_graph(200, 0.08, seed)
+    deg, nbr = csr(adj)
+    g = kplex.Graph.from_csr(deg, nbr)
+    for truss in (0, 1):
+        compare_prep(g, 2, 5, truss)
+
+
+def test_preprocess_rejects_bad_params():
+    g = kplex.Graph.from_edges(4, [(0, 1), (1, 2)])
+    with pytest.raises(kplex.KplexError) as e:
+        kplex.preprocess(g, 3, 5)     # q < 2k
+    assert e.value.code == kplex.KP_EINVAL
+    with pytest.raises(kplex.KplexError):
+        kplex.preprocess(g, 0, 4)
+
+
+def test_relabel_is_a_permutation():
+    g = kplex.Graph.er_planted(300, 0.03, 2, 20, 2, 5)
+    h, perm = g.relabel(99)
+    assert sorted(perm.tolist()) == list(range(300))
+    assert h.validate() == 0 and h.m == g.m
+
+
+def test_kp_enumerate_without_gpu_fails_loudly():
+    try:
+        kplex.device_info()
+        pytest.skip("a GPU is present")
+    except kplex.KplexError:
+        pass
+    g = kplex.Graph.from_edges(6, [(i, j) for i in range(6) for j in range(i + 1, 6)])
+    with pytest.raises(kplex.KplexError) as e:
+        kplex.enumerate_plexes(g, 2, 4)
+    assert e.value.code in (kplex.KP_ENODEV, kplex.KP_EHIP)
