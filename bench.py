@@ -19,21 +19,47 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "maximal-kplex-enumeration_amd"))
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+# name -> (description, generator in kplex.py or None for a shipped dataset, root window or None)
 WORKLOADS = {
-    "cl75k": ("chung-lu n=75000 gamma=2.1 maxdeg=3000 ~508k edges + 300 communities 35-60 @0.9", "workload_cl75k"),
-    "er2000": ("G(2000,0.01) + 20 planted groups of 40", "workload_er2000"),
-    "cl320k": ("chung-lu n=320000 gamma=2.0 ~1.05M edges + 1000 communities 30-80 @0.85, truss", "workload_cl320k"),
+    "cl75k": ("chung-lu n=75000 gamma=2.1 maxdeg=3000 ~508k edges + 300 communities 35-60 @0.9, k=4 q=30; "
+              "roots at degeneracy positions [0,9000) of 16915 (the whole graph holds >1e10 plexes)",
+              "workload_cl75k", (0, 9000)),
+    "cl75k-full": ("chung-lu n=75000 gamma=2.1 maxdeg=3000 ~508k edges + 300 communities 35-60 @0.9, k=4 q=30",
+                   "workload_cl75k", None),
+    "er2000": ("G(2000,0.01) + 20 planted groups of 40, k=3 q=20", "workload_er2000", None),
+    "cl320k": ("chung-lu n=320000 gamma=2.0 ~1.05M edges + 1000 communities 30-80 @0.85, k=5 q=30, truss",
+               "workload_cl320k", None),
+    "soc-epinions": ("reference dataset soc-epinions (README.md:142), k=4 q=30", None, None),
+    "email-euall": ("reference dataset email-euall (README.md:144), k=3 q=12", None, None),
 }
+# published reference results for the shipped datasets: (count, CudaPlex seconds on RTX 6000 Ada)
+PUBLISHED = {"soc-epinions": (13172906, 4.13), "email-euall": (32639016, 1.32)}
+
+
+def dataset_workload(name):
+    import gzip
+    import shutil
+    import tempfile
+    import kplex
+    src = os.path.join(ROOT, "tests", "golden", f"{name}.bin.gz")
+    fd, path = tempfile.mkstemp(suffix=".bin")
+    with gzip.open(src, "rb") as fi, os.fdopen(fd, "wb") as fo:
+        shutil.copyfileobj(fi, fo)
+    g = kplex.Graph.load_bin(path)
+    os.unlink(path)
+    k, q = (4, 30) if name == "soc-epinions" else (3, 12)
+    return g, k, q, 0.25, 0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(graph, k, q, truss, target_s):
+def cpu_baseline(graph, k, q, truss, target_s, window=None):
     """The CPU oracle (OpenMP C restatement of the reference path) on a bounded
-    sample: root positions [0, R) of the degeneracy order, R doubled until the
-    sample has taken >= target_s seconds or covers every root."""
+    sample: root positions [b, R) of the degeneracy order (b = window start),
+    R doubled until the sample has taken >= target_s seconds or covers the
+    window."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import kpo
     deg, nbr = graph.csr()
@@ -41,7 +67,8 @@ def cpu_baseline(graph, k, q, truss, target_s):
     threads = os.cpu_count() or 1
     total_plex = 0
     total_s = 0.0
-    lo, hi = 0, 64
+    wb = window[0] if window else 0
+    lo, hi = wb, wb + 64
     full = None
     while True:
         t0 = time.perf_counter()
@@ -50,6 +77,8 @@ def cpu_baseline(graph, k, q, truss, target_s):
         total_plex += r["count"]
         total_s += dt
         nroots = max(0, r["pn"] - q + 2)
+        if window:
+            nroots = min(nroots, window[1])
         if hi >= nroots:
             full = True
             hi = nroots
@@ -57,10 +86,10 @@ def cpu_baseline(graph, k, q, truss, target_s):
         if total_s >= target_s:
             full = False
             break
-        lo, hi = hi, hi * 2
+        lo, hi = hi, wb + 2 * (hi - wb)
     return {"value": total_plex / total_s if total_s > 0 else 0.0, "unit": "maximal k-plexes/s", "cores": threads,
             "kind": "port",
-            "sample": (f"oracle (OpenMP C restatement) on root positions [0,{hi}) of {nroots}"
+            "sample": (f"oracle (OpenMP C restatement) on root positions [{wb},{hi}) of [{wb},{nroots})"
                        f"{' (all roots)' if full else ''}: {total_plex} plexes in {total_s:.2f} s, "
                        f"preprocessing included per sample chunk")}
 
@@ -81,12 +110,12 @@ def main():
         sys.exit(2)
 
     import kplex
-    desc, fn = WORKLOADS[args.workload]
+    desc, fn, window = WORKLOADS[args.workload]
     log(f"[bench] generating {args.workload}: {desc}")
-    graph, k, q, thres, truss = getattr(kplex, fn)()
+    graph, k, q, thres, truss = getattr(kplex, fn)() if fn else dataset_workload(args.workload)
     log(f"[bench] n={graph.n} m={graph.m} k={k} q={q} truss={truss}; device {kplex.device_info()}")
     t0 = time.perf_counter()
-    prep = kplex.Prepared(graph, k, q, thres, truss, time_limit_s=args.time_limit)
+    prep = kplex.Prepared(graph, k, q, thres, truss, time_limit_s=args.time_limit, roots=window)
     prep_s = time.perf_counter() - t0
     for i in range(args.warmup):
         r = prep.run()
@@ -115,11 +144,12 @@ def main():
         "ms_per_step": step_s * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": None if args.workload not in PUBLISHED else
+        (r0["count"] / step_s) / (PUBLISHED[args.workload][0] / PUBLISHED[args.workload][1]),
         "dtype": "u64",
         "data": "synthetic",
         "config": {"workload": desc, "name": args.workload, "k": k, "q": q, "thres": thres, "truss": truss,
-                   "n": graph.n, "m_directed": graph.m},
+                   "n": graph.n, "m_directed": graph.m, "root_window": list(window) if window else None},
         "count": r0["count"],
         "hash": r0["hash"],
         "tasks": r0["tasks"],
@@ -140,7 +170,7 @@ def main():
     prep.close()
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline (target {args.cpu_target_s} s)")
-        out["cpu_baseline"] = cpu_baseline(graph, k, q, truss, args.cpu_target_s)
+        out["cpu_baseline"] = cpu_baseline(graph, k, q, truss, args.cpu_target_s, window)
     print(json.dumps(out))
 
 

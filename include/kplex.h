@@ -71,6 +71,8 @@ typedef struct kp_params {
     uint32_t flags;        /* KP_FLAG_* */
     double   time_limit_s; /* 0 = none; else KP_ETIMEDOUT after this many seconds of search */
     uint64_t queue_capacity; /* device task-queue slots; 0 = library default */
+    int64_t  root_begin;   /* enumerate only plexes whose first vertex in the degeneracy */
+    int64_t  root_end;     /* order sits at a position in [root_begin, root_end); 0,0 = all */
 } kp_params;
 
 #define KP_FLAG_LIST      1u  /* return every maximal k-plex (original ids, sorted) */

@@ -20,6 +20,10 @@ int check_params(const kp_graph *g, const kp_params *p)
         set_error("invalid parameters k=%d q=%d (need k >= 1 and 2k <= q <= %d)", p->k, p->q, KP_MAX_BLOCK);
         return KP_EINVAL;
     }
+    if (p->root_begin < 0 || p->root_end < 0 || (p->root_end && p->root_end < p->root_begin)) {
+        set_error("invalid root window [%lld, %lld)", (long long)p->root_begin, (long long)p->root_end);
+        return KP_EINVAL;
+    }
     // Cheap structural check, always on: every id indexes a vertex.  (Sorted,
     // symmetric, loop-free lists are assumed as in the reference; the full
     // check is KP_FLAG_VALIDATE.)

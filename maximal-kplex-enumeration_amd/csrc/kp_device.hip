@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -54,15 +55,29 @@ struct RootDesc {          // one valid root in the subgraph arena
 struct Sched {             // persistent-kernel scheduler state (one per launch)
     uint32_t next_root;
     uint32_t active;
-    uint32_t q_count;      // published frames in the queue
-    uint32_t q_size;       // reserved slots
-    uint64_t q_head, q_tail;
+    uint32_t q_count;      // published frames over all sub-queues
+    uint32_t hungry;       // waves currently looking for work
+    uint64_t q_head, q_tail;   // (unused; kept for layout)
     int32_t error;
     uint32_t exited;
     uint64_t count, ha, hb, tasks, alg;
     uint64_t list_used;
     uint64_t donated;
+    uint64_t frames, spills, pops;
+    uint64_t busy_ticks, idle_ticks;   // wall-clock ticks summed over waves
 };
+
+// One shard of the frame queue: a bounded MPMC ring with sequence-numbered
+// slots.  Shards sit on separate 128-byte lines so that waves polling and
+// claiming different shards do not contend on one address.
+struct SubQ {
+    uint32_t count;        // published frames
+    uint32_t size;         // reserved slots
+    uint64_t head, tail;   // tickets
+    uint32_t pad[26];
+};
+static_assert(sizeof(SubQ) == 128, "SubQ is one 128-byte line");
+constexpr uint32_t NSUBQ = 32;
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ uint64_t lt_mask() { return (1ull << __lane_id()) - 1ull; }
@@ -71,15 +86,44 @@ __device__ __forceinline__ uint64_t rfl64(uint64_t x)
 {
     return (uint64_t(rfl(uint32_t(x >> 32))) << 32) | rfl(uint32_t(x));
 }
+// Wave64 reductions: four DPP steps (quad_perm [1,0,3,2], [2,3,0,1],
+// row_half_mirror, row_mirror) reduce each 16-lane row, then the four row
+// results are read out with v_readlane.  The result is wave-uniform.
+#define KP_DPP(v, ctrl) uint32_t(__builtin_amdgcn_update_dpp(0, int(v), ctrl, 0xF, 0xF, false))
 __device__ __forceinline__ uint32_t wmin32(uint32_t v)
 {
-    for (int o = 1; o < 64; o <<= 1) v = min(v, uint32_t(__shfl_xor(int(v), o)));
-    return v;
+    v = min(v, KP_DPP(v, 0xB1));
+    v = min(v, KP_DPP(v, 0x4E));
+    v = min(v, KP_DPP(v, 0x141));
+    v = min(v, KP_DPP(v, 0x140));
+    const uint32_t a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+    const uint32_t c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return min(min(a, b), min(c, d));
+}
+__device__ __forceinline__ uint64_t dpp64(uint64_t v, int ctrl_sel)
+{
+    uint32_t lo = uint32_t(v), hi = uint32_t(v >> 32);
+    switch (ctrl_sel) {
+    case 0: lo = KP_DPP(lo, 0xB1); hi = KP_DPP(hi, 0xB1); break;
+    case 1: lo = KP_DPP(lo, 0x4E); hi = KP_DPP(hi, 0x4E); break;
+    case 2: lo = KP_DPP(lo, 0x141); hi = KP_DPP(hi, 0x141); break;
+    default: lo = KP_DPP(lo, 0x140); hi = KP_DPP(hi, 0x140); break;
+    }
+    return (uint64_t(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, int l)
+{
+    const uint32_t lo = __builtin_amdgcn_readlane(int(uint32_t(x)), l);
+    const uint32_t hi = __builtin_amdgcn_readlane(int(uint32_t(x >> 32)), l);
+    return (uint64_t(hi) << 32) | lo;
 }
 __device__ __forceinline__ uint64_t wsum64(uint64_t v)
 {
-    for (int o = 1; o < 64; o <<= 1) v += (unsigned long long)__shfl_xor((unsigned long long)v, o);
-    return v;
+    v += dpp64(v, 0);
+    v += dpp64(v, 1);
+    v += dpp64(v, 2);
+    v += dpp64(v, 3);
+    return readlane64(v, 0) + readlane64(v, 16) + readlane64(v, 32) + readlane64(v, 48);
 }
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
@@ -101,7 +145,7 @@ struct DecompArgs {
     const int32_t *dpos, *dseq;
     uint32_t pn;
     int k, q;
-    int64_t nroots;
+    int64_t pos_begin, nroots;   // root positions [pos_begin, pos_begin + nroots)
     uint32_t *blk, *lft;
     uint32_t lcap;
     RootInfo *info;
@@ -132,12 +176,13 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
     uint32_t *sb2 = s_b2[wv];
     const int lb2k = a.q - 2 * a.k, bd = a.q - a.k;
 
-    for (int64_t pos = gw; pos < a.nroots; pos += a.nwaves) {
+    for (int64_t slot = gw; slot < a.nroots; slot += a.nwaves) {
+        const int64_t pos = a.pos_begin + slot;
         const uint32_t r = uint32_t(a.dseq[pos]);
         const uint64_t ro = a.off[r];
         const uint32_t dr = a.deg[r];
-        uint32_t *B = a.blk + size_t(pos) * NCAP;
-        uint32_t *L = a.lft + size_t(pos) * a.lcap;
+        uint32_t *B = a.blk + size_t(slot) * NCAP;
+        uint32_t *L = a.lft + size_t(slot) * a.lcap;
         RootInfo info{0, 0, 0, 0, dr};
 
         // 1) split N(r) keeping list (ascending id) order
@@ -164,12 +209,12 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
         }
         info.wr += rfl64(wsum64(wsum));
         if (int(m1) < bd) {   // |B| - 1 < q - k before any filtering
-            if (ln == 0) a.info[pos] = info;
+            if (ln == 0) a.info[slot] = info;
             continue;
         }
         if (m1 > a.d1cap || m1 > 64u * 64u) {
             info.status = -1;
-            if (ln == 0) a.info[pos] = info;
+            if (ln == 0) a.info[slot] = info;
             continue;
         }
         const uint32_t w1 = (m1 + 63) >> 6;
@@ -228,13 +273,13 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
         uint32_t nb1 = 0;
         for (uint32_t t = 0; t < w1; t++) nb1 += __popcll(alive[t]);
         if (int(nb1) < bd) {
-            if (ln == 0) a.info[pos] = info;
+            if (ln == 0) a.info[slot] = info;
             continue;
         }
         const uint32_t hop = nb1 + 1;
         if (hop > NCAP) {
             info.status = -1;
-            if (ln == 0) a.info[pos] = info;
+            if (ln == 0) a.info[slot] = info;
             continue;
         }
         if (ln == 0) B[0] = r;
@@ -334,7 +379,7 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
             }
             fence_agent();
         }
-        if (ln == 0) a.info[pos] = info;
+        if (ln == 0) a.info[slot] = info;
     }
 }
 
@@ -479,14 +524,19 @@ struct SearchArgs {
     uint32_t ndesc;
     const uint64_t *arena;
     int k, q;
-    uint64_t *qframes, *qseq;
-    uint32_t qcap;
+    uint64_t *qframes, *qseq;  // NSUBQ * qcap slots
+    uint32_t qcap;             // slots per shard
+    SubQ *subq;
     Sched *sched;
     uint32_t *list;
     uint64_t list_cap;
-    uint32_t nwaves, stack_cap, donate_period;
-    uint64_t *gstack;
-    uint64_t limit_ticks;   // 0 = no limit; else wall-clock ticks per launch
+    uint64_t *list_used;     // shared by all size-class launches
+    uint32_t nwaves;
+    uint32_t ring_cap;       // frames per wave in the LDS ring
+    uint32_t back_cap;       // frames per wave in the HBM backing stack
+    uint64_t *gstack;        // nwaves * back_cap frames
+    uint32_t donate_period, donate_target;
+    uint64_t limit_ticks;    // 0 = no limit; else wall-clock ticks per launch
 };
 
 enum : uint32_t { FT_BRANCH = 0, FT_KSEARCH = 1 };
@@ -497,28 +547,46 @@ struct Frame {
     uint32_t root, type, r;
 };
 
-template <int W, bool LDS_STACK>
+// frame slot: [meta, P[W], C[W], X[W], D[W], pad] -- an even number of
+// 64-bit words so that slots stay 16-byte aligned
+template <int W>
+struct FrameLayout {
+    static constexpr int FW = (4 * W + 2) & ~1;
+};
+
+template <int W>
+__device__ __forceinline__ uint64_t rdl64(uint64_t x, uint32_t l)
+{
+    const uint32_t lo = __builtin_amdgcn_readlane(int(uint32_t(x)), int(l));
+    const uint32_t hi = __builtin_amdgcn_readlane(int(uint32_t(x >> 32)), int(l));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// LW: left-vertex rows held in registers (left vertices 64t+lane, t < LW)
+template <int W, int LW>
 struct Worker {
-    static constexpr int FW = 4 * W + 1;
+    static constexpr int FW = FrameLayout<W>::FW;
     const SearchArgs &a;
     Sched *S;
-    uint64_t *stk;
+    uint64_t *ring, *back;
     uint32_t ln;
     int k, q;
-    // stack ring (uniform)
-    uint32_t sbase = 0, ssize = 0;
+    // two-level stack (uniform): the newest frames in the LDS ring, older ones in HBM
+    uint32_t rb = 0, rn = 0, bb = 0, bn = 0;
     // current root (uniform)
     uint32_t cur = 0xffffffffu, n = 0, h = 0, l = 0;
-    const uint64_t *A = nullptr, *C2R = nullptr, *C4R = nullptr, *LA = nullptr;
-    // owned rows (per lane)
-    uint64_t adjR[W][W];
+    const uint64_t *LA = nullptr;
+    // rows of the vertices this lane owns (64j + lane), kept in VGPRs
+    uint64_t adjR[W][W], c2R[W][W], c4R[W][W];
+    uint64_t lftR[LW > 0 ? LW : 1][W];
     uint32_t origR[W];
     // accumulators (uniform)
-    uint64_t cnt = 0, ha = 0, hb = 0, tasks = 0, alg = 0;
+    uint64_t cnt = 0, ha = 0, hb = 0, tasks = 0, alg = 0, frames = 0, spills = 0, pops = 0;
     int err = 0;
     uint64_t t_start = 0;
 
-    __device__ Worker(const SearchArgs &args, uint64_t *stack) : a(args), S(args.sched), stk(stack)
+    __device__ Worker(const SearchArgs &args, uint64_t *ring_, uint64_t *back_)
+        : a(args), S(args.sched), ring(ring_), back(back_)
     {
         ln = __lane_id();
         k = a.k;
@@ -529,89 +597,159 @@ struct Worker {
     {
         return a.limit_ticks && wall_clock64() - t_start > a.limit_ticks;
     }
-
     __device__ __forceinline__ void set_error(int code)
     {
         if (ln == 0) atomicCAS(&S->error, 0, code);
         err = code;
     }
 
-    // ---- frames ----
-    __device__ __forceinline__ void write_frame(uint64_t *dst, const Frame<W> &f)
+    // ---- frame slots (only lane 0 touches them; values broadcast with readfirstlane) ----
+    __device__ __forceinline__ void store_slot(uint64_t *dst, uint64_t meta, const uint64_t *P, const uint64_t *C,
+                                               const uint64_t *X, const uint64_t *D, bool withD)
     {
         if (ln == 0) {
+            dst[0] = meta;
 #pragma unroll
             for (int w = 0; w < W; w++) {
-                dst[w] = f.P[w];
-                dst[W + w] = f.C[w];
-                dst[2 * W + w] = f.D[w];
-                dst[3 * W + w] = f.X[w];
+                dst[1 + w] = P[w];
+                dst[1 + W + w] = C[w];
+                dst[1 + 2 * W + w] = X[w];
+                if (withD) dst[1 + 3 * W + w] = D[w];
             }
-            dst[4 * W] = uint64_t(f.root) | (uint64_t(f.type) << 32) | (uint64_t(f.r) << 40);
         }
     }
-    __device__ __forceinline__ void read_frame(const uint64_t *src, Frame<W> &f)
+    __device__ __forceinline__ void load_slot(const uint64_t *src, Frame<W> &f)
     {
-        uint64_t tmp[FW];
+        uint64_t meta = 0;
+        uint64_t t[4 * W];
         if (ln == 0) {
+            meta = src[0];
 #pragma unroll
-            for (int i = 0; i < FW; i++) tmp[i] = src[i];
+            for (int i = 0; i < 3 * W; i++) t[i] = src[1 + i];
         }
-#pragma unroll
-        for (int w = 0; w < W; w++) {
-            f.P[w] = rfl64(tmp[w]);
-            f.C[w] = rfl64(tmp[W + w]);
-            f.D[w] = rfl64(tmp[2 * W + w]);
-            f.X[w] = rfl64(tmp[3 * W + w]);
-        }
-        const uint64_t meta = rfl64(tmp[4 * W]);
+        meta = rfl64(meta);
         f.root = uint32_t(meta);
         f.type = uint32_t(meta >> 32) & 0xff;
         f.r = uint32_t(meta >> 40) & 0xff;
+        if (f.type == FT_KSEARCH && ln == 0) {
+#pragma unroll
+            for (int i = 0; i < W; i++) t[3 * W + i] = src[1 + 3 * W + i];
+        }
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            f.P[w] = rfl64(t[w]);
+            f.C[w] = rfl64(t[W + w]);
+            f.X[w] = rfl64(t[2 * W + w]);
+            f.D[w] = f.type == FT_KSEARCH ? rfl64(t[3 * W + w]) : 0ull;
+        }
+    }
+    static __device__ __forceinline__ uint64_t make_meta(uint32_t root, uint32_t type, uint32_t r)
+    {
+        return uint64_t(root) | (uint64_t(type) << 32) | (uint64_t(r) << 40);
+    }
+    __device__ __forceinline__ void copy_slot(uint64_t *dst, const uint64_t *src)
+    {
+        if (ln == 0) {
+#pragma unroll
+            for (int i = 0; i < FW; i++) dst[i] = src[i];
+        }
+    }
+    __device__ __forceinline__ uint64_t *ring_slot(uint32_t i) const
+    {
+        uint32_t s = rb + i;
+        if (s >= a.ring_cap) s -= a.ring_cap;
+        return ring + size_t(s) * FW;
+    }
+    __device__ __forceinline__ uint64_t *back_slot(uint32_t i) const
+    {
+        uint32_t s = bb + i;
+        if (s >= a.back_cap) s -= a.back_cap;
+        return back + size_t(s) * FW;
+    }
+    __device__ __forceinline__ uint32_t depth() const { return rn + bn; }
+
+    __device__ bool push(uint64_t meta, const uint64_t *P, const uint64_t *C, const uint64_t *X, const uint64_t *D,
+                         bool withD)
+    {
+        if (rn == a.ring_cap) {   // spill the oldest ring frame to the HBM backing stack
+            if (bn == a.back_cap) {
+                set_error(KP_EOVERFLOW);
+                return false;
+            }
+            copy_slot(back_slot(bn), ring_slot(0));
+            bn++;
+            rb = (rb + 1 == a.ring_cap) ? 0 : rb + 1;
+            rn--;
+            spills++;
+        }
+        store_slot(ring_slot(rn), meta, P, C, X, D, withD);
+        rn++;
+        return true;
     }
     __device__ __forceinline__ bool push(const Frame<W> &f)
     {
-        if (ssize >= a.stack_cap) {
-            set_error(KP_EOVERFLOW);
-            return false;
+        return push(make_meta(f.root, f.type, f.r), f.P, f.C, f.X, f.D, f.type == FT_KSEARCH);
+    }
+    __device__ void pop_top(Frame<W> &f)
+    {
+        if (rn) {
+            rn--;
+            load_slot(ring_slot(rn), f);
+        } else {
+            bn--;
+            load_slot(back_slot(bn), f);
         }
-        uint32_t slot = sbase + ssize;
-        if (slot >= a.stack_cap) slot -= a.stack_cap;
-        write_frame(stk + size_t(slot) * FW, f);
-        ssize++;
-        return true;
     }
-    __device__ __forceinline__ void pop_top(Frame<W> &f)
+    // the oldest frame: the largest pending subtree
+    __device__ void pop_bottom(Frame<W> &f, bool &from_back)
     {
-        ssize--;
-        uint32_t slot = sbase + ssize;
-        if (slot >= a.stack_cap) slot -= a.stack_cap;
-        read_frame(stk + size_t(slot) * FW, f);
+        from_back = bn > 0;
+        if (from_back) {
+            load_slot(back_slot(0), f);
+            bb = (bb + 1 == a.back_cap) ? 0 : bb + 1;
+            bn--;
+        } else {
+            load_slot(ring_slot(0), f);
+            rb = (rb + 1 == a.ring_cap) ? 0 : rb + 1;
+            rn--;
+        }
     }
-    __device__ __forceinline__ void pop_bottom(Frame<W> &f)
+    __device__ void unpop_bottom(bool from_back)
     {
-        read_frame(stk + size_t(sbase) * FW, f);
-        sbase = (sbase + 1 == a.stack_cap) ? 0 : sbase + 1;
-        ssize--;
+        if (from_back) {
+            bb = bb == 0 ? a.back_cap - 1 : bb - 1;
+            bn++;
+        } else {
+            rb = rb == 0 ? a.ring_cap - 1 : rb - 1;
+            rn++;
+        }
     }
 
-    // ---- global MPMC queue (bounded, sequence-numbered slots) ----
+    // ---- sharded MPMC frame queue ----
+    __device__ __forceinline__ uint32_t home_shard() const { return (blockIdx.x * 4u + (threadIdx.x >> 6)) % NSUBQ; }
+
     __device__ bool q_push(const Frame<W> &f)
     {
-        uint32_t ok = 0;
+        uint32_t ok = 0, shard = 0;
         uint64_t t = 0;
         if (ln == 0) {
-            const uint32_t s = atomicAdd(&S->q_size, 1u);
-            if (s >= a.qcap) {
-                atomicSub(&S->q_size, 1u);
-            } else {
-                ok = 1;
-                t = atomicAdd((unsigned long long *)&S->q_tail, 1ull);
+            const uint32_t home = home_shard();
+            for (uint32_t p = 0; p < NSUBQ && !ok; p++) {
+                const uint32_t j = (home + p) % NSUBQ;
+                const uint32_t s = atomicAdd(&a.subq[j].size, 1u);
+                if (s >= a.qcap) {
+                    atomicSub(&a.subq[j].size, 1u);
+                } else {
+                    ok = 1;
+                    shard = j;
+                    t = atomicAdd((unsigned long long *)&a.subq[j].tail, 1ull);
+                }
             }
         }
         if (!rfl(ok)) return false;
+        shard = rfl(shard);
         t = rfl64(t);
-        const uint32_t slot = uint32_t(t % a.qcap);
+        const size_t slot = size_t(shard) * a.qcap + uint32_t(t % a.qcap);
         uint32_t bad = 0;
         if (ln == 0) {
             uint32_t spins = 0;
@@ -624,31 +762,39 @@ struct Worker {
             set_error(KP_ETIMEDOUT);
             return false;
         }
-        write_frame(a.qframes + size_t(slot) * FW, f);
+        store_slot(a.qframes + slot * FW, make_meta(f.root, f.type, f.r), f.P, f.C, f.X, f.D, f.type == FT_KSEARCH);
         if (ln == 0) {
             __hip_atomic_store(&a.qseq[slot], t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&S->q_count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&a.subq[shard].count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&S->q_count, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
         }
         return true;
     }
+    // The caller has already counted itself in S->active.
     __device__ bool q_pop(Frame<W> &f)
     {
-        uint32_t ok = 0;
+        uint32_t ok = 0, shard = 0;
         uint64_t hd = 0;
         if (ln == 0) {
-            uint32_t c = __hip_atomic_load(&S->q_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            while (c > 0) {
-                if (__hip_atomic_compare_exchange_strong(&S->q_count, &c, c - 1, __ATOMIC_ACQUIRE,
-                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    ok = 1;
-                    break;
+            const uint32_t home = home_shard();
+            for (uint32_t p = 0; p < NSUBQ && !ok; p++) {
+                const uint32_t j = (home + p) % NSUBQ;
+                uint32_t c = __hip_atomic_load(&a.subq[j].count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while (c > 0) {
+                    if (__hip_atomic_compare_exchange_strong(&a.subq[j].count, &c, c - 1, __ATOMIC_ACQUIRE,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        ok = 1;
+                        shard = j;
+                        break;
+                    }
                 }
             }
-            if (ok) hd = atomicAdd((unsigned long long *)&S->q_head, 1ull);
+            if (ok) hd = atomicAdd((unsigned long long *)&a.subq[shard].head, 1ull);
         }
         if (!rfl(ok)) return false;
+        shard = rfl(shard);
         hd = rfl64(hd);
-        const uint32_t slot = uint32_t(hd % a.qcap);
+        const size_t slot = size_t(shard) * a.qcap + uint32_t(hd % a.qcap);
         uint32_t bad = 0;
         if (ln == 0) {
             uint32_t spins = 0;
@@ -661,15 +807,17 @@ struct Worker {
             set_error(KP_ETIMEDOUT);
             return false;
         }
-        read_frame(a.qframes + size_t(slot) * FW, f);
+        load_slot(a.qframes + slot * FW, f);
         if (ln == 0) {
             __hip_atomic_store(&a.qseq[slot], hd + a.qcap, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            atomicSub(&S->q_size, 1u);
+            atomicSub(&a.subq[shard].size, 1u);
+            __hip_atomic_fetch_sub(&S->q_count, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
         }
+        pops++;
         return true;
     }
 
-    // ---- root cache ----
+    // ---- root cache: this lane's rows of the current root ----
     __device__ void load_root(uint32_t ridx)
     {
         const RootDesc *d = a.desc + ridx;
@@ -677,20 +825,42 @@ struct Worker {
         h = rfl(d->h);
         l = rfl(d->l);
         const uint64_t off = rfl64(d->off);
-        A = a.arena + off;
-        C2R = A + size_t(n) * W;
-        C4R = C2R + size_t(n) * W;
-        LA = C4R + size_t(n) * W;
+        const uint64_t *A = a.arena + off;
+        const uint64_t *C2 = A + size_t(n) * W;
+        const uint64_t *C4 = C2 + size_t(n) * W;
+        LA = C4 + size_t(n) * W;
         const uint32_t *og = reinterpret_cast<const uint32_t *>(LA + size_t(l) * W);
 #pragma unroll
         for (int j = 0; j < W; j++) {
             const uint32_t v = 64 * j + ln;
             const bool in = v < n;
 #pragma unroll
-            for (int w = 0; w < W; w++) adjR[j][w] = in ? A[size_t(v) * W + w] : 0ull;
+            for (int w = 0; w < W; w++) {
+                adjR[j][w] = in ? A[size_t(v) * W + w] : 0ull;
+                c2R[j][w] = in ? C2[size_t(v) * W + w] : 0ull;
+                c4R[j][w] = in ? C4[size_t(v) * W + w] : 0ull;
+            }
             origR[j] = in ? og[v] : 0u;
         }
+#pragma unroll
+        for (int t = 0; t < LW; t++) {
+            const uint32_t u = 64 * t + ln;
+#pragma unroll
+            for (int w = 0; w < W; w++) lftR[t][w] = u < l ? LA[size_t(u) * W + w] : 0ull;
+        }
         cur = ridx;
+    }
+
+    // row of vertex v (uniform) out of the owner lane's registers
+    __device__ __forceinline__ void row_of(const uint64_t (&R)[W][W], uint32_t v, uint64_t *out) const
+    {
+        const uint32_t pw = v >> 6, pl = v & 63;
+#pragma unroll
+        for (int j = 0; j < W; j++)
+            if (uint32_t(j) == pw) {
+#pragma unroll
+                for (int w = 0; w < W; w++) out[w] = rdl64<W>(R[j][w], pl);
+            }
     }
 
     // ---- per-vertex helpers ----
@@ -716,13 +886,8 @@ struct Worker {
         for (int w = 0; w < W; w++) c += __popcll(Sx[w]);
         return c;
     }
-    __device__ __forceinline__ void load_row(const uint64_t *base, uint32_t v, uint64_t *out) const
-    {
-#pragma unroll
-        for (int w = 0; w < W; w++) out[w] = rfl64(base[size_t(v) * W + w]);
-    }
 
-    // upperBound: every P vertex keeps >= q-k neighbours in Sx
+    // upperBound (:1002-1040): every P vertex keeps >= q-k neighbours in Sx
     __device__ __forceinline__ bool ub_ok(const uint64_t *P, const uint64_t *Sx) const
     {
         bool fail = false;
@@ -753,7 +918,19 @@ struct Worker {
         uint64_t Sat[W];
 #pragma unroll
         for (int j = 0; j < W; j++) Sat[j] = __ballot(has(Sx, j) && int(deg_in(j, Sx)) + k < int(s) + 1);
-        for (uint32_t t = 0; t < l; t += 64) {
+#pragma unroll
+        for (int t = 0; t < LW; t++) {
+            if (64u * t >= l) return true;
+            int c = 0;
+            uint64_t bad = 0;
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                c += __popcll(lftR[t][w] & Sx[w]);
+                bad |= Sat[w] & ~lftR[t][w];
+            }
+            if (__ballot(64u * t + ln < l && bad == 0 && c + k >= int(s) + 1)) return false;
+        }
+        for (uint32_t t = 64u * LW; t < l; t += 64) {
             const uint32_t u = t + ln;
             bool ext = false;
             if (u < l) {
@@ -788,7 +965,7 @@ struct Worker {
         hb += mix64(sb + uint64_t(s) * kp::HASH_LB);
         if (a.list) {
             uint64_t base = 0;
-            if (ln == 0) base = atomicAdd((unsigned long long *)&S->list_used, (unsigned long long)(s + 1));
+            if (ln == 0) base = atomicAdd((unsigned long long *)a.list_used, (unsigned long long)(s + 1));
             base = rfl64(base);
             if (base + s + 1 > a.list_cap) {
                 set_error(KP_EOVERFLOW);
@@ -807,14 +984,16 @@ struct Worker {
     __device__ __forceinline__ void count_task() { tasks++; alg += 2ull * (5ull * n + 16ull); }
 
     // ---- Branch frame (BNB :1516-1724) ----
-    __device__ void branch(const Frame<W> &f)
+    // The exclude child goes on the stack; the include child, when it exists,
+    // is returned in nf and processed next without a trip through the stack.
+    __device__ bool branch(const Frame<W> &f, Frame<W> &nf)
     {
         const uint64_t *P = f.P, *C = f.C, *X = f.X;
-        const uint32_t ps = popc(P), cs = popc(C), xs = popc(X);
-        if (int(ps + cs) < q) return;
+        const uint32_t ps = popc(P), cs = popc(C);
+        if (int(ps + cs) < q) return false;
         if (cs == 0) {
-            if (xs == 0 && int(ps) >= q && maximal(P, ps)) record(P, ps);
-            return;
+            if (popc(X) == 0 && int(ps) >= q && maximal(P, ps)) record(P, ps);
+            return false;
         }
         uint64_t PC[W];
 #pragma unroll
@@ -832,11 +1011,11 @@ struct Worker {
         key = rfl(wmin32(key));
         const int minP = int(key >> 10);
         const uint32_t pp = key & 1023u;
-        if (minP + k < max(q, int(ps))) return;
+        if (minP + k < max(q, int(ps))) return false;
         uint32_t pivot;
         if (minP + k < int(ps + cs)) {
             uint64_t adjPP[W];
-            load_row(A, pp, adjPP);
+            row_of(adjR, pp, adjPP);
             uint32_t k2 = ~0u;
 #pragma unroll
             for (int j = 0; j < W; j++)
@@ -844,7 +1023,7 @@ struct Worker {
             k2 = rfl(wmin32(k2));
             if (k2 == ~0u) {   // unreachable while the degrees are consistent
                 set_error(KP_EINVAL);
-                return;
+                return false;
             }
             pivot = k2 & 1023u;
         } else {
@@ -864,158 +1043,145 @@ struct Worker {
 #pragma unroll
                 for (int j = 0; j < W; j++)
                     if (has(X, j) && int(dG[j]) + k >= int(tot) + 1 && subset_of_adj(j, Sat)) ext = true;
-                if (__ballot(ext)) return;
+                if (__ballot(ext)) return false;
                 if (maximal(PC, tot)) record(PC, tot);
-                return;
+                return false;
             }
             pivot = k2 & 1023u;
         }
         // branchInCand2 (:1467-1472): exclude child, then include child
         const uint32_t pw = pivot >> 6;
         const uint64_t pb = 1ull << (pivot & 63);
-        Frame<W> ex;
+        uint64_t C0[W], XE[W];
 #pragma unroll
         for (int w = 0; w < W; w++) {
-            ex.P[w] = P[w];
-            ex.C[w] = C[w] & ~(w == int(pw) ? pb : 0ull);
-            ex.D[w] = 0;
-            ex.X[w] = X[w] | (w == int(pw) ? pb : 0ull);
+            C0[w] = C[w] & ~(w == int(pw) ? pb : 0ull);
+            XE[w] = X[w] | (w == int(pw) ? pb : 0ull);
         }
-        ex.root = f.root;
-        ex.type = FT_BRANCH;
-        ex.r = 0;
         count_task();
-        if (!push(ex)) return;
+        const uint64_t meta = make_meta(f.root, FT_BRANCH, 0);
+        if (!push(meta, P, C0, XE, nullptr, false)) return false;
         // include child (:1273-1465)
-        Frame<W> in;
+        uint64_t P2[W], OK[W], c2v[W], c4v[W];
 #pragma unroll
-        for (int w = 0; w < W; w++) in.P[w] = P[w] | (w == int(pw) ? pb : 0ull);
-        const uint32_t ps2 = ps + 1;
-        uint64_t OK[W], c2v[W], c4v[W];
-        compat_mask(in.P, ps2, OK);
-        load_row(C2R, pivot, c2v);
-        load_row(C4R, pivot, c4v);
-        uint64_t PC2[W];
+        for (int w = 0; w < W; w++) P2[w] = P[w] | (w == int(pw) ? pb : 0ull);
+        compat_mask(P2, ps + 1, OK);
+        row_of(c2R, pivot, c2v);
+        uint64_t C2[W], PC2[W];
 #pragma unroll
         for (int w = 0; w < W; w++) {
-            in.C[w] = ex.C[w] & c2v[w] & OK[w];
-            PC2[w] = in.P[w] | in.C[w];
+            C2[w] = C0[w] & c2v[w] & OK[w];
+            PC2[w] = P2[w] | C2[w];
         }
-        if (!ub_ok(in.P, PC2)) return;
+        if (!ub_ok(P2, PC2)) return false;
+        row_of(c4R, pivot, c4v);
 #pragma unroll
         for (int w = 0; w < W; w++) {
-            in.X[w] = X[w] & c4v[w] & OK[w];
-            in.D[w] = 0;
+            nf.P[w] = P2[w];
+            nf.C[w] = C2[w];
+            nf.X[w] = X[w] & c4v[w] & OK[w];
+            nf.D[w] = 0;
         }
-        in.root = f.root;
-        in.type = FT_BRANCH;
-        in.r = 0;
+        nf.root = f.root;
+        nf.type = FT_BRANCH;
+        nf.r = 0;
         count_task();
-        push(in);
+        return true;
     }
 
-    // emit a Branch task from the k-search (state-0 emit, :2005-2082)
-    __device__ __forceinline__ void emit0(const Frame<W> &f, const uint64_t *X)
+    // emit a Branch task from the k-search (state-0 emit, :2005-2082) as the next frame
+    __device__ __forceinline__ bool emit0(uint32_t root, const uint64_t *P, const uint64_t *C, const uint64_t *X,
+                                          Frame<W> &nf)
     {
-        const uint32_t ps = popc(f.P), cs = popc(f.C);
-        if (int(ps + cs) < q) return;
+        const uint32_t ps = popc(P), cs = popc(C);
+        if (int(ps + cs) < q) return false;
         if (ps > 1) {
             uint64_t PC[W];
 #pragma unroll
-            for (int w = 0; w < W; w++) PC[w] = f.P[w] | f.C[w];
-            if (!ub_ok(f.P, PC)) return;
+            for (int w = 0; w < W; w++) PC[w] = P[w] | C[w];
+            if (!ub_ok(P, PC)) return false;
         }
-        Frame<W> b;
+        count_task();
 #pragma unroll
         for (int w = 0; w < W; w++) {
-            b.P[w] = f.P[w];
-            b.C[w] = f.C[w];
-            b.D[w] = 0;
-            b.X[w] = X[w];
+            nf.P[w] = P[w];
+            nf.C[w] = C[w];
+            nf.X[w] = X[w];
+            nf.D[w] = 0;
         }
-        b.root = f.root;
-        b.type = FT_BRANCH;
-        b.r = 0;
-        count_task();
-        push(b);
+        nf.root = root;
+        nf.type = FT_BRANCH;
+        nf.r = 0;
+        return true;
     }
 
-    // ---- k-search frame: enumerate S ⊆ C2 with |S| <= budget ----
-    __device__ void ksearch(const Frame<W> &f)
+    // ---- k-search frame: enumerate S ⊆ C2 with |S| <= budget (:1855-2444) ----
+    // The exclude child goes on the stack; the include child is the next frame.
+    __device__ bool ksearch(const Frame<W> &f, Frame<W> &nf)
     {
         int top = -1;
 #pragma unroll
         for (int w = W - 1; w >= 0; w--)
             if (top < 0 && f.D[w]) top = 64 * w + 63 - __clzll(f.D[w]);
-        if (top < 0) {
-            emit0(f, f.X);
-            return;
-        }
+        if (top < 0) return emit0(f.root, f.P, f.C, f.X, nf);
         if (f.r == 0) {   // no budget: every C2 vertex is excluded
             uint64_t XD[W];
 #pragma unroll
             for (int w = 0; w < W; w++) XD[w] = f.X[w] | f.D[w];
-            emit0(f, XD);
-            return;
+            return emit0(f.root, f.P, f.C, XD, nf);
         }
         const uint32_t u = uint32_t(top);
         const uint32_t uw = u >> 6;
         const uint64_t ub = 1ull << (u & 63);
-        Frame<W> ex;
+        uint64_t D1[W], X1[W];
 #pragma unroll
         for (int w = 0; w < W; w++) {
-            ex.P[w] = f.P[w];
-            ex.C[w] = f.C[w];
-            ex.D[w] = f.D[w] & ~(w == int(uw) ? ub : 0ull);
-            ex.X[w] = f.X[w] | (w == int(uw) ? ub : 0ull);
+            D1[w] = f.D[w] & ~(w == int(uw) ? ub : 0ull);
+            X1[w] = f.X[w] | (w == int(uw) ? ub : 0ull);
         }
-        ex.root = f.root;
-        ex.type = FT_KSEARCH;
-        ex.r = f.r;
-        if (!push(ex)) return;
+        if (!push(make_meta(f.root, FT_KSEARCH, f.r), f.P, f.C, X1, D1, true)) return false;
         uint64_t c2u[W];
-        load_row(C2R, u, c2u);
-        Frame<W> in;
+        row_of(c2R, u, c2u);
 #pragma unroll
         for (int w = 0; w < W; w++) {
-            in.P[w] = f.P[w] | (w == int(uw) ? ub : 0ull);
-            in.C[w] = f.C[w] & c2u[w];                           // updateCand13
-            in.D[w] = f.D[w] & ~(w == int(uw) ? ub : 0ull) & c2u[w];   // updateCand23
-            in.X[w] = f.X[w];
+            nf.P[w] = f.P[w] | (w == int(uw) ? ub : 0ull);
+            nf.C[w] = f.C[w] & c2u[w];      // updateCand13
+            nf.D[w] = D1[w] & c2u[w];       // updateCand23
+            nf.X[w] = f.X[w];
         }
-        in.root = f.root;
+        nf.root = f.root;
         if (f.r == 1) {
             // state-4 emit (:2255-2393): bounds, then C1 filtered by isKplex2
-            const uint32_t ps = popc(in.P), cs = popc(in.C);
-            if (int(ps + cs) < q) return;
+            const uint32_t ps = popc(nf.P), cs = popc(nf.C);
+            if (int(ps + cs) < q) return false;
             if (ps > 1) {
                 uint64_t PC[W];
 #pragma unroll
-                for (int w = 0; w < W; w++) PC[w] = in.P[w] | in.C[w];
-                if (!ub_ok(in.P, PC)) return;
+                for (int w = 0; w < W; w++) PC[w] = nf.P[w] | nf.C[w];
+                if (!ub_ok(nf.P, PC)) return false;
             }
             uint64_t OK[W];
-            compat_mask(in.P, ps, OK);
+            compat_mask(nf.P, ps, OK);
 #pragma unroll
             for (int w = 0; w < W; w++) {
-                in.C[w] &= OK[w];
-                in.D[w] = 0;
+                nf.C[w] &= OK[w];
+                nf.D[w] = 0;
             }
-            in.type = FT_BRANCH;
-            in.r = 0;
+            nf.type = FT_BRANCH;
+            nf.r = 0;
             count_task();
-            push(in);
         } else {
-            in.type = FT_KSEARCH;
-            in.r = f.r - 1;
-            push(in);
+            nf.type = FT_KSEARCH;
+            nf.r = f.r - 1;
         }
+        return true;
     }
 
-    __device__ void root_frame(uint32_t ridx, Frame<W> &f)
+    __device__ void push_root(uint32_t ridx)
     {
         const RootDesc *d = a.desc + ridx;
-        const uint32_t rn = rfl(d->n), rh = rfl(d->h);
+        const uint32_t rn_ = rfl(d->n), rh = rfl(d->h);
+        uint64_t P[W], C[W], D[W], X[W];
 #pragma unroll
         for (int w = 0; w < W; w++) {
             const int lo = 64 * w;
@@ -1025,14 +1191,12 @@ struct Worker {
                 const uint64_t hi = t == 64 ? ~0ull : ((1ull << t) - 1ull);
                 return hi & ~((1ull << s) - 1ull);
             };
-            f.P[w] = range(0, 1);
-            f.C[w] = range(1, int(rh));
-            f.D[w] = range(int(rh), int(rn));
-            f.X[w] = 0;
+            P[w] = range(0, 1);
+            C[w] = range(1, int(rh));
+            D[w] = range(int(rh), int(rn_));
+            X[w] = 0;
         }
-        f.root = ridx;
-        f.type = FT_KSEARCH;
-        f.r = uint32_t(k - 1);
+        push(make_meta(ridx, FT_KSEARCH, uint32_t(k - 1)), P, C, X, D, true);
     }
 
     // Termination: a wave counts itself in S->active only while it holds
@@ -1040,100 +1204,124 @@ struct Worker {
     // (seq_cst), so "no roots left, queue empty, nobody active", read in that
     // order, means no frame exists anywhere.  A wave that reads it exits; a
     // wave that still holds work never exits before its stack is empty.
+    // Waves without work count themselves in S->hungry; busy waves donate
+    // their oldest frames (the largest pending subtrees) while hungry waves
+    // outnumber the queued frames.
     __device__ void run()
     {
-        bool active_self = false;
+        bool active_self = false, hungry_self = false, have = false;
         uint32_t since_donate = 0, since_check = 0, backoff = 1;
-        Frame<W> f;
+        uint64_t t_mark = t_start, busy_t = 0, idle_t = 0;
+        const uint32_t gw = blockIdx.x * 4u + (threadIdx.x >> 6);
+        Frame<W> f, nf;
         for (;;) {
-            if (ssize == 0) {
-                if (active_self) {
-                    if (ln == 0) __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                    active_self = false;
-                }
-                if (err) break;
-                bool got = false;
-                uint32_t qc = 0, nr = 0;
-                if (ln == 0) {
-                    qc = __hip_atomic_load(&S->q_count, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                    nr = __hip_atomic_load(&S->next_root, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                qc = rfl(qc);
-                nr = rfl(nr);
-                if (qc > 0) {
-                    if (ln == 0) __hip_atomic_fetch_add(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                    got = q_pop(f);
-                    if (got) {
-                        active_self = true;
-                        push(f);
-                    } else if (ln == 0) {
-                        __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+            if (!have) {
+                if (depth() == 0) {
+                    if (active_self) {
+                        const uint64_t now = wall_clock64();
+                        busy_t += now - t_mark;
+                        t_mark = now;
+                        if (ln == 0) __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                        active_self = false;
                     }
                     if (err) break;
-                }
-                if (!got && nr < a.ndesc) {
-                    uint32_t rr = 0xffffffffu;
+                    bool got = false;
+                    uint32_t qc = 0, nr = 0;
                     if (ln == 0) {
-                        __hip_atomic_fetch_add(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                        rr = __hip_atomic_fetch_add(&S->next_root, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                        if (rr >= a.ndesc)
+                        qc = __hip_atomic_load(&S->q_count, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                        nr = __hip_atomic_load(&S->next_root, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    qc = rfl(qc);
+                    nr = rfl(nr);
+                    if (qc > 0) {
+                        if (ln == 0) __hip_atomic_fetch_add(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                        got = q_pop(f);
+                        if (got) {
+                            active_self = true;
+                            have = true;
+                        } else if (ln == 0) {
                             __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        if (err) break;
                     }
-                    rr = rfl(rr);
-                    if (rr < a.ndesc) {
-                        active_self = true;
-                        root_frame(rr, f);
-                        push(f);
-                        got = true;
+                    if (!got && nr < a.ndesc) {
+                        uint32_t rr = 0xffffffffu;
+                        if (ln == 0) {
+                            __hip_atomic_fetch_add(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                            rr = __hip_atomic_fetch_add(&S->next_root, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                            if (rr >= a.ndesc)
+                                __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        rr = rfl(rr);
+                        if (rr < a.ndesc) {
+                            active_self = true;
+                            push_root(rr);
+                            got = true;
+                        }
                     }
+                    if (!got) {
+                        if (!hungry_self) {
+                            if (ln == 0) atomicAdd(&S->hungry, 1u);
+                            hungry_self = true;
+                        }
+                        uint32_t done = 0, stop = 0;
+                        if (ln == 0) {
+                            const uint32_t nr2 = __hip_atomic_load(&S->next_root, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                            const uint32_t qc2 = __hip_atomic_load(&S->q_count, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                            const uint32_t ac = __hip_atomic_load(&S->active, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                            done = nr2 >= a.ndesc && qc2 == 0 && ac == 0;
+                            stop = __hip_atomic_load(&S->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+                            if (past_deadline()) stop = 2;
+                        }
+                        done = rfl(done);
+                        stop = rfl(stop);
+                        if (stop == 2) set_error(KP_ETIMEDOUT);
+                        if (done || stop) break;
+                        // jittered exponential backoff: waves do not poll in lockstep
+                        const uint32_t jit = (gw * 2654435761u + backoff * 97u) >> 28;
+                        for (uint32_t z = 0; z < backoff + jit; z++) __builtin_amdgcn_s_sleep(16);
+                        if (backoff < 16) backoff <<= 1;
+                        continue;
+                    }
+                    if (hungry_self) {
+                        if (ln == 0) atomicSub(&S->hungry, 1u);
+                        hungry_self = false;
+                    }
+                    backoff = 1;
+                    const uint64_t now = wall_clock64();
+                    idle_t += now - t_mark;
+                    t_mark = now;
                 }
-                if (!got) {
-                    uint32_t done = 0, stop = 0;
-                    if (ln == 0) {
-                        const uint32_t nr2 = __hip_atomic_load(&S->next_root, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                        const uint32_t qc2 = __hip_atomic_load(&S->q_count, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                        const uint32_t ac = __hip_atomic_load(&S->active, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                        done = nr2 >= a.ndesc && qc2 == 0 && ac == 0;
-                        stop = __hip_atomic_load(&S->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-                        if (past_deadline()) stop = 2;
-                    }
-                    done = rfl(done);
-                    stop = rfl(stop);
-                    if (stop == 2) set_error(KP_ETIMEDOUT);
-                    if (done || stop) break;
-                    for (uint32_t z = 0; z < backoff; z++) __builtin_amdgcn_s_sleep(127);
-                    if (backoff < 16) backoff <<= 1;
-                    continue;
-                }
-                backoff = 1;
+                if (!have) pop_top(f);
             }
-            pop_top(f);
+            frames++;
             if (f.root != cur) load_root(f.root);
-            if (f.type == FT_KSEARCH) ksearch(f);
-            else branch(f);
+            have = (f.type == FT_KSEARCH) ? ksearch(f, nf) : branch(f, nf);
             if (err) break;
+            if (have) f = nf;
             if (++since_donate >= a.donate_period) {
                 since_donate = 0;
-                if (ssize >= 2) {
+                const uint32_t avail = depth() + (have ? 1u : 0u);
+                if (avail >= 2 && depth() >= 1) {
                     uint32_t want = 0;
                     if (ln == 0) {
+                        const uint32_t hg = __hip_atomic_load(&S->hungry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         const uint32_t qc = __hip_atomic_load(&S->q_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        const uint32_t ac = __hip_atomic_load(&S->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        const uint32_t nr = __hip_atomic_load(&S->next_root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        want = nr >= a.ndesc && ac < a.nwaves && qc < a.nwaves - ac;
+                        want = hg > qc ? min(hg - qc, 4u) : 0u;
                     }
-                    if (rfl(want)) {
+                    want = rfl(want);
+                    for (uint32_t z = 0; z < want && depth() >= 1 && depth() + (have ? 1u : 0u) >= 2; z++) {
                         Frame<W> b;
-                        pop_bottom(b);
+                        bool from_back;
+                        pop_bottom(b, from_back);
                         if (q_push(b)) {
                             if (ln == 0) atomicAdd((unsigned long long *)&S->donated, 1ull);
                         } else {
-                            // queue full: the frame goes back to the bottom of the stack
-                            sbase = sbase == 0 ? a.stack_cap - 1 : sbase - 1;
-                            ssize++;
-                            if (err) break;
+                            unpop_bottom(from_back);   // queue full: the frame stays where it was
+                            break;
                         }
                     }
+                    if (err) break;
                 }
             }
             if (++since_check >= 1024) {
@@ -1149,34 +1337,47 @@ struct Worker {
             }
         }
         if (active_self && ln == 0) __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+        if (hungry_self && ln == 0) atomicSub(&S->hungry, 1u);
+        {
+            const uint64_t now = wall_clock64();
+            if (active_self) busy_t += now - t_mark;
+            else idle_t += now - t_mark;
+        }
         if (ln == 0) {
+            atomicAdd((unsigned long long *)&S->busy_ticks, (unsigned long long)busy_t);
+            atomicAdd((unsigned long long *)&S->idle_ticks, (unsigned long long)idle_t);
             atomicAdd((unsigned long long *)&S->count, (unsigned long long)cnt);
             atomicAdd((unsigned long long *)&S->ha, (unsigned long long)ha);
             atomicAdd((unsigned long long *)&S->hb, (unsigned long long)hb);
             atomicAdd((unsigned long long *)&S->tasks, (unsigned long long)tasks);
             atomicAdd((unsigned long long *)&S->alg, (unsigned long long)alg);
+            atomicAdd((unsigned long long *)&S->frames, (unsigned long long)frames);
+            atomicAdd((unsigned long long *)&S->spills, (unsigned long long)spills);
+            atomicAdd((unsigned long long *)&S->pops, (unsigned long long)pops);
             atomicAdd(&S->exited, 1u);
         }
     }
 };
 
-template <int W, bool LDS_STACK>
+template <int W, int LW>
 __global__ __launch_bounds__(256) void search_kernel(SearchArgs a)
 {
     extern __shared__ uint64_t lds[];
-    constexpr int FW = 4 * W + 1;
+    constexpr int FW = FrameLayout<W>::FW;
     const int wv = threadIdx.x >> 6;
     const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + wv;
     if (gw >= a.nwaves) return;
-    uint64_t *stk = LDS_STACK ? lds + size_t(wv) * a.stack_cap * FW : a.gstack + size_t(gw) * a.stack_cap * FW;
-    Worker<W, LDS_STACK> wk(a, stk);
+    uint64_t *ring = lds + size_t(wv) * a.ring_cap * FW;
+    uint64_t *back = a.gstack ? a.gstack + size_t(gw) * a.back_cap * FW : nullptr;
+    Worker<W, LW> wk(a, ring, back);
     wk.run();
 }
 
-__global__ void iota_kernel(uint64_t *seq, uint32_t n)
+// slot j of every shard starts with sequence number j (free for ticket j)
+__global__ void iota_kernel(uint64_t *seq, uint32_t n, uint32_t per_shard)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) seq[i] = i;
+    if (i < n) seq[i] = i % per_shard;
 }
 
 } // namespace kpd
@@ -1250,13 +1451,24 @@ struct kp_prepared {
     // graph in HBM
     DevBuf off, nbr, deg, dpos, dseq, orig;
     // per-run buffers (reused across runs)
-    DevBuf blk, lft, info, ws_cnt, ws_b1c, ws_a1, desc, arena, wr_extra, qframes, qseq, sched, gstack, list;
+    DevBuf blk, lft, info, ws_cnt, ws_b1c, ws_a1, desc, arena, wr_extra, list, list_used;
+    // one scheduler / queue / backing stack per size class (their kernels run concurrently)
+    DevBuf cq[kNumClass], cseq[kNumClass], csched[kNumClass], cstack[kNumClass], csubq[kNumClass];
+    hipStream_t cstream[kNumClass] = {};
     uint32_t lcap = 0;
     ~kp_prepared()
     {
         DevBuf *all[] = {&off, &nbr, &deg, &dpos, &dseq, &orig, &blk, &lft, &info, &ws_cnt, &ws_b1c, &ws_a1,
-                         &desc, &arena, &wr_extra, &qframes, &qseq, &sched, &gstack, &list};
+                         &desc, &arena, &wr_extra, &list, &list_used};
         for (DevBuf *b : all) b->release();
+        for (int c = 0; c < kNumClass; c++) {
+            cq[c].release();
+            cseq[c].release();
+            csched[c].release();
+            cstack[c].release();
+            csubq[c].release();
+            if (cstream[c]) (void)hipStreamDestroy(cstream[c]);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -1282,10 +1494,11 @@ uint64_t dev_bytes(const kp_prepared *h)
 {
     if (!h) return 0;
     const DevBuf *all[] = {&h->off, &h->nbr, &h->deg, &h->dpos, &h->dseq, &h->orig, &h->blk, &h->lft, &h->info,
-                           &h->ws_cnt, &h->ws_b1c, &h->ws_a1, &h->desc, &h->arena, &h->wr_extra, &h->qframes,
-                           &h->qseq, &h->sched, &h->gstack, &h->list};
+                           &h->ws_cnt, &h->ws_b1c, &h->ws_a1, &h->desc, &h->arena, &h->wr_extra, &h->list, &h->list_used};
     uint64_t s = 0;
     for (const DevBuf *b : all) s += b->bytes;
+    for (int c = 0; c < kNumClass; c++)
+        s += h->cq[c].bytes + h->cseq[c].bytes + h->csched[c].bytes + h->cstack[c].bytes + h->csubq[c].bytes;
     return s;
 }
 
@@ -1307,6 +1520,8 @@ int dev_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out)
     if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) { delete h; return KP_EHIP; }
     h->cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return KP_EHIP; }
+    for (int c = 0; c < kNumClass; c++)
+        if (hipStreamCreateWithFlags(&h->cstream[c], hipStreamNonBlocking) != hipSuccess) { delete h; return KP_EHIP; }
 
     const double t0 = now_ms();
     int rc = preprocess(g, p->k, p->q, p->truss, &h->prep);
@@ -1360,48 +1575,68 @@ int launch_build(kp_prepared *h, const BuildArgs &ba, uint32_t nwg)
     return KP_OK;
 }
 
-template <int W, bool LDSS>
-int launch_search(kp_prepared *h, SearchArgs sa, uint32_t maxn, Sched *host_sched)
+struct ClassLaunch {
+    int W = 0;
+    uint32_t nwaves = 0, ring = 0, back = 0;
+    int blocks_per_cu = 0;
+};
+
+template <int W, int LW>
+int launch_search(kp_prepared *h, int c, SearchArgs sa, uint32_t maxn, ClassLaunch *cl)
 {
-    constexpr int FW = 4 * W + 1;
-    sa.stack_cap = maxn + 3;
-    const size_t per_wave = size_t(sa.stack_cap) * FW * 8;
-    const size_t dyn = LDSS ? per_wave * 4 : 0;
-    auto kern = search_kernel<W, LDSS>;
+    constexpr int FW = FrameLayout<W>::FW;
+    const uint32_t need = maxn + 3;   // DFS depth bound: every child shrinks C2 or C
+    const uint32_t ring = std::min<uint32_t>(need, std::max<uint32_t>(16, 8192 / (FW * 8)));
+    sa.ring_cap = ring;
+    sa.back_cap = need > ring ? need - ring + 1 : 0;
+    const size_t dyn = size_t(ring) * FW * 8 * 4;
+    auto kern = search_kernel<W, LW>;
     if (dyn > 64 * 1024) HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn)));
     int nb = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, dyn));
     if (nb < 1) nb = 1;
     const uint32_t nwg = uint32_t(nb) * uint32_t(h->cus);
     sa.nwaves = nwg * 4;
-    sa.qcap = std::max<uint32_t>(4096, sa.nwaves * 4);
-    sa.donate_period = 4;
+    sa.qcap = std::max<uint32_t>(256, sa.nwaves * 4 / NSUBQ);   // slots per shard
+    sa.donate_period = 8;
+    sa.donate_target = std::max<uint32_t>(16, sa.nwaves / 4);
+    const size_t nslots = size_t(sa.qcap) * NSUBQ;
     int rc;
-    if ((rc = h->qframes.ensure(size_t(sa.qcap) * FW * 8)) || (rc = h->qseq.ensure(size_t(sa.qcap) * 8))) return rc;
-    if (!LDSS && (rc = h->gstack.ensure(size_t(sa.nwaves) * per_wave))) return rc;
-    sa.qframes = h->qframes.as<uint64_t>();
-    sa.qseq = h->qseq.as<uint64_t>();
-    sa.gstack = LDSS ? nullptr : h->gstack.as<uint64_t>();
-    HIPCHK(hipMemsetAsync(sa.sched, 0, sizeof(Sched), h->stream));
-    hipLaunchKernelGGL(iota_kernel, dim3((sa.qcap + 255) / 256), dim3(256), 0, h->stream, sa.qseq, sa.qcap);
+    if ((rc = h->cq[c].ensure(nslots * FW * 8)) || (rc = h->cseq[c].ensure(nslots * 8)) ||
+        (rc = h->csched[c].ensure(sizeof(Sched))) || (rc = h->csubq[c].ensure(sizeof(SubQ) * NSUBQ)))
+        return rc;
+    if (sa.back_cap && (rc = h->cstack[c].ensure(size_t(sa.nwaves) * sa.back_cap * FW * 8))) return rc;
+    sa.qframes = h->cq[c].as<uint64_t>();
+    sa.qseq = h->cseq[c].as<uint64_t>();
+    sa.sched = h->csched[c].as<Sched>();
+    sa.subq = h->csubq[c].as<SubQ>();
+    sa.gstack = sa.back_cap ? h->cstack[c].as<uint64_t>() : nullptr;
+    hipStream_t st = h->cstream[c];
+    HIPCHK(hipMemsetAsync(sa.sched, 0, sizeof(Sched), st));
+    HIPCHK(hipMemsetAsync(sa.subq, 0, sizeof(SubQ) * NSUBQ, st));
+    hipLaunchKernelGGL(iota_kernel, dim3(uint32_t((nslots + 255) / 256)), dim3(256), 0, st, sa.qseq, uint32_t(nslots),
+                       sa.qcap);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(kern, dim3(nwg), dim3(256), dyn, h->stream, sa);
+    hipLaunchKernelGGL(kern, dim3(nwg), dim3(256), dyn, st, sa);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(host_sched, sa.sched, sizeof(Sched), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    cl->W = W;
+    cl->nwaves = sa.nwaves;
+    cl->ring = sa.ring_cap;
+    cl->back = sa.back_cap;
+    cl->blocks_per_cu = nb;
     return KP_OK;
 }
 
-int run_search_class(kp_prepared *h, int c, SearchArgs sa, uint32_t maxn, Sched *hs)
+int run_search_class(kp_prepared *h, int c, SearchArgs sa, uint32_t maxn, ClassLaunch *cl)
 {
     switch (kClassW[c]) {
-    case 1: return launch_search<1, true>(h, sa, maxn, hs);
-    case 2: return launch_search<2, true>(h, sa, maxn, hs);
-    case 3: return launch_search<3, true>(h, sa, maxn, hs);
-    case 4: return launch_search<4, false>(h, sa, maxn, hs);
-    case 6: return launch_search<6, false>(h, sa, maxn, hs);
-    case 8: return launch_search<8, false>(h, sa, maxn, hs);
-    case 16: return launch_search<16, false>(h, sa, maxn, hs);
+    case 1: return launch_search<1, 2>(h, c, sa, maxn, cl);
+    case 2: return launch_search<2, 2>(h, c, sa, maxn, cl);
+    case 3: return launch_search<3, 2>(h, c, sa, maxn, cl);
+    case 4: return launch_search<4, 1>(h, c, sa, maxn, cl);
+    case 6: return launch_search<6, 0>(h, c, sa, maxn, cl);
+    case 8: return launch_search<8, 0>(h, c, sa, maxn, cl);
+    case 16: return launch_search<16, 0>(h, c, sa, maxn, cl);
     }
     return KP_EINVAL;
 }
@@ -1432,7 +1667,15 @@ int dev_run(kp_prepared *h, kp_result *res)
     res->pm = h->pm;
     res->preprocess_ms = h->preprocess_ms;
     res->upload_ms = h->upload_ms;
-    const int64_t nroots = std::max<int64_t>(0, int64_t(h->pn) - P.q + 2);
+    // roots at degeneracy positions [0, pn - q + 2) (src/device_funcs.cu:44-45), optionally windowed
+    int64_t pos_end = std::max<int64_t>(0, int64_t(h->pn) - P.q + 2);
+    int64_t pos_begin = 0;
+    if (P.root_end > 0 || P.root_begin > 0) {
+        pos_begin = std::min<int64_t>(P.root_begin, pos_end);
+        if (P.root_end > 0) pos_end = std::min<int64_t>(P.root_end, pos_end);
+        pos_end = std::max(pos_end, pos_begin);
+    }
+    const int64_t nroots = pos_end - pos_begin;
     hipEvent_t e0, e1, e2;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -1469,6 +1712,7 @@ int dev_run(kp_prepared *h, kp_result *res)
         da.pn = std::max<uint32_t>(h->pn, 1);
         da.k = P.k;
         da.q = P.q;
+        da.pos_begin = pos_begin;
         da.nroots = nroots;
         da.blk = h->blk.as<uint32_t>();
         da.lft = h->lft.as<uint32_t>();
@@ -1491,10 +1735,10 @@ int dev_run(kp_prepared *h, kp_result *res)
             const RootInfo &ri = info[size_t(pos)];
             if (ri.status < 0) {
                 if (ri.status == -1) {
-                    kp::set_error("root at degeneracy position %lld: block exceeds %u vertices", (long long)pos, NCAP);
+                    kp::set_error("root at degeneracy position %lld: block exceeds %u vertices", (long long)(pos + pos_begin), NCAP);
                     return KP_ERANGE;
                 }
-                kp::set_error("root at degeneracy position %lld: left set exceeds %u vertices", (long long)pos, h->lcap);
+                kp::set_error("root at degeneracy position %lld: left set exceeds %u vertices", (long long)(pos + pos_begin), h->lcap);
                 return KP_EOVERFLOW;
             }
             if (ri.status != 1) continue;
@@ -1562,33 +1806,72 @@ int dev_run(kp_prepared *h, kp_result *res)
         res->alg_bytes += 4 * wr + (uint64_t(ri.n) * ri.n + 7) / 8 + uint64_t(ri.n) * ri.n;
     }
 
-    // ---- persistent search, one launch per size class ----
+    // ---- persistent search: one launch per size class, all classes concurrently ----
     const bool want_list = (P.flags & KP_FLAG_LIST) != 0;
     const uint64_t list_cap = want_list ? (64ull << 20) : 0;
     if (want_list && (rc = h->list.ensure(list_cap * 4))) return rc;
-    if ((rc = h->sched.ensure(sizeof(Sched)))) return rc;
-    std::vector<uint32_t> listed;
+    if ((rc = h->list_used.ensure(8))) return rc;
+    HIPCHK(hipMemsetAsync(h->list_used.p, 0, 8, h->stream));
     uint64_t limit_ticks = 0;
     if (P.time_limit_s > 0) {
         int khz = 0;
         HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device));
         limit_ticks = uint64_t(P.time_limit_s * double(khz) * 1e3);
     }
-    for (int c = 0; c < kNumClass; c++) {
+    // every class stream starts after the build; the main stream waits for all of them
+    hipEvent_t ebuild;
+    HIPCHK(hipEventCreateWithFlags(&ebuild, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ebuild, h->stream));
+    std::vector<int> order;
+    for (int c = 0; c < kNumClass; c++)
+        if (class_begin[c + 1] > class_begin[c]) order.push_back(c);
+    // largest classes (by sum of n^2 over their roots) first
+    std::vector<double> cwork(kNumClass, 0.0);
+    for (size_t i = 0; i < descs.size(); i++) cwork[dclass[i]] += double(descs[i].n) * descs[i].n;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cwork[x] > cwork[y]; });
+    ClassLaunch launches[kNumClass];
+    std::vector<hipEvent_t> cdone;
+    const bool serial_classes = std::getenv("KP_SERIAL_CLASSES") != nullptr;
+    int launch_rc = KP_OK;
+    for (int c : order) {
         const uint32_t b = class_begin[c], e = class_begin[c + 1];
-        if (b == e) continue;
+        HIPCHK(hipStreamWaitEvent(h->cstream[c], ebuild, 0));
         SearchArgs sa{};
         sa.desc = h->desc.as<RootDesc>() + b;
         sa.ndesc = e - b;
         sa.arena = h->arena.as<uint64_t>();
         sa.k = P.k;
         sa.q = P.q;
-        sa.sched = h->sched.as<Sched>();
         sa.list = want_list ? h->list.as<uint32_t>() : nullptr;
         sa.list_cap = list_cap;
+        sa.list_used = h->list_used.as<uint64_t>();
         sa.limit_ticks = limit_ticks;
+        if ((launch_rc = run_search_class(h, c, sa, class_maxn[c], &launches[c]))) break;
+        hipEvent_t ev;
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(ev, h->cstream[c]));
+        HIPCHK(hipStreamWaitEvent(h->stream, ev, 0));
+        cdone.push_back(ev);
+        if (serial_classes) HIPCHK(hipStreamSynchronize(h->cstream[c]));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int c : order) HIPCHK(hipStreamSynchronize(h->cstream[c]));
+    for (hipEvent_t ev : cdone) (void)hipEventDestroy(ev);
+    (void)hipEventDestroy(ebuild);
+    if (launch_rc) return launch_rc;
+    for (int c : order) {
         Sched hs{};
-        if ((rc = run_search_class(h, c, sa, class_maxn[c], &hs))) return rc;
+        HIPCHK(hipMemcpy(&hs, h->csched[c].p, sizeof(Sched), hipMemcpyDeviceToHost));
+        if (std::getenv("KP_STATS"))
+            std::fprintf(stderr,
+                         "[kp] search W=%d roots=%u waves=%u (%d blocks/CU) ring=%u back=%u frames=%llu tasks=%llu "
+                         "donated=%llu pops=%llu spills=%llu busy=%.1f%% wave-s=%.1f\n",
+                         kClassW[c], class_begin[c + 1] - class_begin[c], launches[c].nwaves,
+                         launches[c].blocks_per_cu, launches[c].ring, launches[c].back, (unsigned long long)hs.frames,
+                         (unsigned long long)hs.tasks, (unsigned long long)hs.donated, (unsigned long long)hs.pops,
+                         (unsigned long long)hs.spills,
+                         100.0 * double(hs.busy_ticks) / double(std::max<uint64_t>(1, hs.busy_ticks + hs.idle_ticks)),
+                         double(hs.busy_ticks + hs.idle_ticks) / 1e8);
         if (hs.error) {
             kp::set_error("search kernel (class W=%d) reported error %d", kClassW[c], hs.error);
             return hs.error;
@@ -1599,11 +1882,17 @@ int dev_run(kp_prepared *h, kp_result *res)
         res->tasks += hs.tasks;
         res->alg_bytes += hs.alg;
         res->search_alg_bytes += hs.alg;
-        if (want_list && hs.list_used) {
-            const size_t old = listed.size();
-            listed.resize(old + hs.list_used);
-            HIPCHK(hipMemcpy(listed.data() + old, h->list.p, 4 * hs.list_used, hipMemcpyDeviceToHost));
+    }
+    std::vector<uint32_t> listed;
+    if (want_list) {
+        uint64_t used = 0;
+        HIPCHK(hipMemcpy(&used, h->list_used.p, 8, hipMemcpyDeviceToHost));
+        if (used > list_cap) {
+            kp::set_error("plex list exceeds %llu words", (unsigned long long)list_cap);
+            return KP_EOVERFLOW;
         }
+        listed.resize(used);
+        if (used) HIPCHK(hipMemcpy(listed.data(), h->list.p, 4 * used, hipMemcpyDeviceToHost));
     }
     HIPCHK(hipEventRecord(e2, h->stream));
     HIPCHK(hipEventSynchronize(e2));

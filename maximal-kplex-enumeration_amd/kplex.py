@@ -38,7 +38,8 @@ class KpGraph(ctypes.Structure):
 
 class KpParams(ctypes.Structure):
     _fields_ = [("k", c_int), ("q", c_int), ("thres", c_float), ("truss", c_int), ("flags", c_uint32),
-                ("time_limit_s", c_double), ("queue_capacity", c_uint64)]
+                ("time_limit_s", c_double), ("queue_capacity", c_uint64), ("root_begin", ctypes.c_int64),
+                ("root_end", ctypes.c_int64)]
 
 
 class KpResult(ctypes.Structure):
@@ -232,16 +233,18 @@ def _result_dict(r, listed):
     return d
 
 
-def make_params(k, q, thres=0.25, truss=0, flags=0, time_limit_s=0.0, queue_capacity=0):
+def make_params(k, q, thres=0.25, truss=0, flags=0, time_limit_s=0.0, queue_capacity=0, roots=None):
+    rb, re_ = roots if roots else (0, 0)
     return KpParams(k=k, q=q, thres=thres, truss=truss, flags=flags, time_limit_s=time_limit_s,
-                    queue_capacity=queue_capacity)
+                    queue_capacity=queue_capacity, root_begin=rb, root_end=re_)
 
 
-def enumerate_plexes(graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0, validate=False):
-    """kp_enumerate: returns a dict with count, hash, tasks, timings (and plexes when listing)."""
+def enumerate_plexes(graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0, validate=False, roots=None):
+    """kp_enumerate: returns a dict with count, hash, tasks, timings (and plexes when listing).
+    roots=(b, e) restricts the enumeration to roots at degeneracy positions [b, e)."""
     lib = load()
     flags = (KP_FLAG_LIST if listing else 0) | (KP_FLAG_VALIDATE if validate else 0)
-    p = make_params(k, q, thres, truss, flags, time_limit_s)
+    p = make_params(k, q, thres, truss, flags, time_limit_s, roots=roots)
     r = KpResult()
     _check(lib.kp_enumerate(ctypes.byref(graph.g), ctypes.byref(p), ctypes.byref(r)), "kp_enumerate")
     try:
@@ -253,11 +256,11 @@ def enumerate_plexes(graph, k, q, thres=0.25, truss=0, listing=False, time_limit
 class Prepared:
     """kp_prepare / kp_run: the graph preprocessed and resident in HBM."""
 
-    def __init__(self, graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0):
+    def __init__(self, graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0, roots=None):
         self.lib = load()
         self.h = c_void_p()
         self.listing = listing
-        p = make_params(k, q, thres, truss, KP_FLAG_LIST if listing else 0, time_limit_s)
+        p = make_params(k, q, thres, truss, KP_FLAG_LIST if listing else 0, time_limit_s, roots=roots)
         _check(self.lib.kp_prepare(ctypes.byref(graph.g), ctypes.byref(p), ctypes.byref(self.h)), "kp_prepare")
 
     def run(self):

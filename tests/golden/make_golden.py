@@ -26,11 +26,13 @@ OUT = os.path.join(ROOT, "tests", "golden", "oracle_results.json")
 KEYS = ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l", "alg_bytes", "adj_bytes")
 
 
-def run(og, k, q, truss, listing=False):
+def run(og, k, q, truss, listing=False, window=None):
     t0 = time.time()
-    r = kpo.enumerate_plexes(og, k, q, truss, threads=os.cpu_count(), listing=listing)
+    wb, we = window if window else (0, -1)
+    r = kpo.enumerate_plexes(og, k, q, truss, threads=os.cpu_count(), listing=listing, pos_begin=wb, pos_end=we)
     d = {x: r[x] for x in KEYS}
-    d.update(k=k, q=q, truss=truss, oracle_s=round(time.time() - t0, 3), threads=os.cpu_count())
+    d.update(k=k, q=q, truss=truss, oracle_s=round(time.time() - t0, 3), threads=os.cpu_count(),
+             window=list(window) if window else None)
     if listing:
         d["plexes"] = [list(p) for p in r["plexes"]]
     return d
@@ -43,7 +45,8 @@ def from_graph(g):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--with-large", action="store_true", help="also the Chung-Lu 75k workload")
+    ap.add_argument("--with-large", action="store_true",
+                    help="also the Chung-Lu 75k root windows [4000,6000) and [0,9000) (minutes of oracle time)")
     args = ap.parse_args()
     gold = {}
     if os.path.exists(OUT):
@@ -64,7 +67,9 @@ def main():
         shutil.rmtree(tmp)
     if args.with_large:
         g, k, q, _, tr = kplex.workload_cl75k()
-        gold["cl75k"] = run(from_graph(g), k, q, tr)
+        og = from_graph(g)
+        gold["cl75k_4000_6000"] = run(og, k, q, tr, window=(4000, 6000))
+        gold["cl75k_0_9000"] = run(og, k, q, tr, window=(0, 9000))
     with open(OUT, "w") as f:
         json.dump(gold, f, indent=1)
     print(json.dumps({n: {x: v[x] for x in ("count", "hash", "tasks", "oracle_s")} for n, v in gold.items()}, indent=1))

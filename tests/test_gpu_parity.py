@@ -117,12 +117,30 @@ def test_readme_known_answers(datasets, gold, case):
     same(r, gold[case], ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l"))
 
 
-def test_cl75k_against_golden(gold):
-    if "cl75k" not in gold:
-        pytest.skip("no recorded oracle result for the Chung-Lu 75k workload")
+@pytest.mark.parametrize("window", [(4000, 6000), (0, 9000)])
+def test_cl75k_window_against_golden(gold, window):
+    # the Chung-Lu 75k workload restricted to a window of root positions
+    key = f"cl75k_{window[0]}_{window[1]}"
+    if key not in gold:
+        pytest.skip(f"no recorded oracle result for {key}")
     g, k, q, _, tr = kplex.workload_cl75k()
-    r = gpu(g, k, q, tr, name="cl75k")
-    same(r, gold["cl75k"], ("count", "hash", "tasks", "valid_roots", "pn", "pm"))
+    say(f"{key} n={g.n} m={g.m}")
+    r = kplex.enumerate_plexes(g, k, q, truss=tr, time_limit_s=LIMIT, roots=window)
+    same(r, gold[key], ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l"))
+
+
+def test_root_windows_partition_the_result():
+    # windows [0,a) and [a,end) add up to the whole enumeration, in count, hash and tasks
+    g = kplex.Graph.er_planted(1200, 0.01, 6, 30, 3, 9)
+    say("windows")
+    whole = kplex.enumerate_plexes(g, 3, 14, time_limit_s=LIMIT)
+    a = kplex.enumerate_plexes(g, 3, 14, time_limit_s=LIMIT, roots=(0, 400))
+    b = kplex.enumerate_plexes(g, 3, 14, time_limit_s=LIMIT, roots=(400, 1 << 30))
+    assert a["count"] + b["count"] == whole["count"]
+    assert a["tasks"] + b["tasks"] == whole["tasks"]
+    m = (1 << 64) - 1
+    assert (a["hash_a"] + b["hash_a"]) & m == whole["hash_a"]
+    assert (a["hash_b"] + b["hash_b"]) & m == whole["hash_b"]
 
 
 def test_relabel_invariance():
