@@ -701,10 +701,9 @@ struct Worker {
         }
     }
     // the oldest frame: the largest pending subtree
-    __device__ void pop_bottom(Frame<W> &f, bool &from_back)
+    __device__ void pop_bottom(Frame<W> &f)
     {
-        from_back = bn > 0;
-        if (from_back) {
+        if (bn > 0) {
             load_slot(back_slot(0), f);
             bb = (bb + 1 == a.back_cap) ? 0 : bb + 1;
             bn--;
@@ -714,21 +713,13 @@ struct Worker {
             rn--;
         }
     }
-    __device__ void unpop_bottom(bool from_back)
-    {
-        if (from_back) {
-            bb = bb == 0 ? a.back_cap - 1 : bb - 1;
-            bn++;
-        } else {
-            rb = rb == 0 ? a.ring_cap - 1 : rb - 1;
-            rn++;
-        }
-    }
 
     // ---- sharded MPMC frame queue ----
     __device__ __forceinline__ uint32_t home_shard() const { return (blockIdx.x * 4u + (threadIdx.x >> 6)) % NSUBQ; }
 
-    __device__ bool q_push(const Frame<W> &f)
+    // Reserves a free slot in some shard (home shard first).  Returns the
+    // global slot index, or ~0 when every shard is full.
+    __device__ size_t q_reserve(uint32_t &shard_out, uint64_t &ticket_out)
     {
         uint32_t ok = 0, shard = 0;
         uint64_t t = 0;
@@ -746,7 +737,7 @@ struct Worker {
                 }
             }
         }
-        if (!rfl(ok)) return false;
+        if (!rfl(ok)) return ~size_t(0);
         shard = rfl(shard);
         t = rfl64(t);
         const size_t slot = size_t(shard) * a.qcap + uint32_t(t % a.qcap);
@@ -760,15 +751,20 @@ struct Worker {
         }
         if (rfl(bad)) {
             set_error(KP_ETIMEDOUT);
-            return false;
+            return ~size_t(0);
         }
+        shard_out = shard;
+        ticket_out = t;
+        return slot;
+    }
+    __device__ void q_publish(size_t slot, uint32_t shard, uint64_t t, const Frame<W> &f)
+    {
         store_slot(a.qframes + slot * FW, make_meta(f.root, f.type, f.r), f.P, f.C, f.X, f.D, f.type == FT_KSEARCH);
         if (ln == 0) {
             __hip_atomic_store(&a.qseq[slot], t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_add(&a.subq[shard].count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_add(&S->q_count, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
         }
-        return true;
     }
     // The caller has already counted itself in S->active.
     __device__ bool q_pop(Frame<W> &f)
@@ -1311,15 +1307,14 @@ struct Worker {
                     }
                     want = rfl(want);
                     for (uint32_t z = 0; z < want && depth() >= 1 && depth() + (have ? 1u : 0u) >= 2; z++) {
+                        uint32_t shard;
+                        uint64_t t;
+                        const size_t slot = q_reserve(shard, t);
+                        if (slot == ~size_t(0)) break;   // queue full (or error): keep the frames
                         Frame<W> b;
-                        bool from_back;
-                        pop_bottom(b, from_back);
-                        if (q_push(b)) {
-                            if (ln == 0) atomicAdd((unsigned long long *)&S->donated, 1ull);
-                        } else {
-                            unpop_bottom(from_back);   // queue full: the frame stays where it was
-                            break;
-                        }
+                        pop_bottom(b);
+                        q_publish(slot, shard, t, b);
+                        if (ln == 0) atomicAdd((unsigned long long *)&S->donated, 1ull);
                     }
                     if (err) break;
                 }
