@@ -51,6 +51,19 @@ def dataset_workload(name):
     return g, k, q, 0.25, 0
 
 
+def profiled_traffic(name):
+    """HBM bytes per search-kernel launch measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (copy-calibrated) for this workload, from its committed summary."""
+    path = os.path.join(ROOT, "profiles", f"search_kernel_{name}_window.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -163,7 +176,7 @@ def main():
         "full_pipeline_s": (r0["preprocess_ms"] + r0["upload_ms"]) / 1e3 + step_s,
         "adjacency_GBps": r0["adj_bytes"] / step_s / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.workload),
                      "kernel": "search_kernel", "alg_bytes_per_launch": r0["search_alg_bytes"]},
         "cpu_baseline": None,
     }
