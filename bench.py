@@ -107,6 +107,39 @@ def cpu_baseline(graph, k, q, truss, target_s, window=None):
                        f"preprocessing included per sample chunk")}
 
 
+def batched_vs_plain(ngraphs, time_limit):
+    """The many-graph workload (Chung-Lu exponent 2.3, average degree 10, n
+    log-uniform in [200, 20000], 1-20 planted 25-50-vertex communities at 0.9,
+    k in {2,3,4}, q = 3k+8), first `ngraphs` graphs of the 4,096: every graph
+    through the plain entry point one after another (kp_enumerate: preprocess,
+    upload, decompose, search per graph), then all of them through one
+    kp_enumerate_batch call.  Results must be identical graph by graph."""
+    import kplex
+    specs = kplex.batch_graph_specs()[:ngraphs]
+    graphs = [kplex.graph_from_spec(s) for s in specs]
+    params = [(s["k"], s["q"]) for s in specs]
+    log(f"[bench] batched: {len(graphs)} graphs, n {min(s['n'] for s in specs)}..{max(s['n'] for s in specs)}")
+    kplex.enumerate_plexes(graphs[0], *params[0], time_limit_s=time_limit)   # warm the library
+    t0 = time.perf_counter()
+    plain = [kplex.enumerate_plexes(g, k, q, time_limit_s=time_limit) for g, (k, q) in zip(graphs, params)]
+    plain_s = time.perf_counter() - t0
+    log(f"[bench] batched: plain {plain_s:.2f} s")
+    t0 = time.perf_counter()
+    batch = kplex.enumerate_batch(graphs, [dict(k=k, q=q, time_limit_s=time_limit) for k, q in params])
+    batch_s = time.perf_counter() - t0
+    log(f"[bench] batched: one batch {batch_s:.2f} s")
+    for a, b in zip(plain, batch):
+        for x in ("count", "hash", "tasks", "valid_roots"):
+            assert a[x] == b[x], ("batched result differs", x, a[x], b[x])
+    count = sum(r["count"] for r in batch)
+    return {"graphs": len(graphs), "plexes": count, "tasks": sum(r["tasks"] for r in batch),
+            "plain_s": plain_s, "batched_s": batch_s, "plain_plexes_per_s": count / plain_s,
+            "batched_plexes_per_s": count / batch_s, "batched_vs_plain": plain_s / batch_s,
+            "plain_gpu_s": sum(r["decompose_ms"] + r["search_ms"] for r in plain) / 1e3,
+            "batched_gpu_s": (batch[0]["decompose_ms"] + batch[0]["search_ms"]) / 1e3,
+            "identical": True}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +149,8 @@ def main():
     ap.add_argument("--time-limit", type=float, default=300.0)
     ap.add_argument("--cpu-target-s", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-graphs", type=int, default=48,
+                    help="graphs of the many-graph workload for the batched-vs-plain line (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -184,6 +219,8 @@ def main():
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline (target {args.cpu_target_s} s)")
         out["cpu_baseline"] = cpu_baseline(graph, k, q, truss, args.cpu_target_s, window)
+    if args.batch_graphs > 0:
+        out["batched"] = batched_vs_plain(args.batch_graphs, args.time_limit)
     print(json.dumps(out))
 
 

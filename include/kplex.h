@@ -66,13 +66,18 @@ void kp_graph_free(kp_graph *g);
 typedef struct kp_params {
     int      k;            /* -k : at most k-1 non-neighbours per vertex inside the plex */
     int      q;            /* -q : minimum plex size (reference global lb) */
-    float    thres;        /* -t : pause threshold, fraction of the device task-queue capacity */
+    float    thres;        /* -t : fraction of queue_capacity kept on the device as resumable
+                              frames after a pause; the rest is spilled to pinned host memory */
     int      truss;        /* -tr: 1 = (q-2k)-truss prefilter before the core */
     uint32_t flags;        /* KP_FLAG_* */
     double   time_limit_s; /* 0 = none; else KP_ETIMEDOUT after this many seconds of search */
-    uint64_t queue_capacity; /* device task-queue slots; 0 = library default */
+    uint64_t queue_capacity; /* device task capacity in frames per size class; 0 = 4,194,304
+                              (the reference's MAX_CAP) */
     int64_t  root_begin;   /* enumerate only plexes whose first vertex in the degeneracy */
     int64_t  root_end;     /* order sits at a position in [root_begin, root_end); 0,0 = all */
+    uint64_t pause_every;  /* 0 = never; else each search launch pauses after about this many
+                              frames, its pending frames are carried (device / pinned host)
+                              and a new launch resumes them.  Results do not change. */
 } kp_params;
 
 #define KP_FLAG_LIST      1u  /* return every maximal k-plex (original ids, sorted) */
@@ -136,7 +141,10 @@ void kp_prepared_free(kp_prepared *h);
 
 /* ---- batched: many graphs in one persistent launch ------------------ */
 /* No reference counterpart (the reference handles one graph per process).
- * p may hold one kp_params per graph (stride = sizeof(kp_params)). */
+ * p[i] holds graph i's k, q, truss and root window; flags, time limit and
+ * pause settings are taken from p[0].  All roots of all graphs share one set
+ * of size-class launches; r[i] is graph i's result (decompose/search times are
+ * the batch's). */
 int  kp_enumerate_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r);
 
 /* ---- served: continuous batching of (graph, k, q) queries ----------- */

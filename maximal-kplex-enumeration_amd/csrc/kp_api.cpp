@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 
 using namespace kp;
 
@@ -94,6 +95,20 @@ extern "C" int kp_enumerate(const kp_graph *g, const kp_params *p, kp_result *r)
     rc = dev_run(h, r);
     dev_free(h);
     return rc;
+}
+
+extern "C" int kp_enumerate_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r)
+{
+    if (n && (!g || !p || !r)) return KP_EINVAL;
+    for (size_t i = 0; i < n; i++) {
+        int rc = check_params(g[i], &p[i]);
+        if (rc) {
+            const std::string why = last_error();
+            set_error("graph %zu: %s", i, why.c_str());
+            return rc;
+        }
+    }
+    return dev_batch(g, p, n, r);
 }
 
 extern "C" void kp_result_free(kp_result *r)

@@ -1,11 +1,13 @@
 // kp_device.hip -- MI355X (gfx950) kernels and device driver of libkplex.
 //
-// Pipeline per (graph, k, q):
-//   decompose_kernel   one wave64 per root (grid-stride): later/earlier split of
-//                      N(r), (q-2k)-core of the later 1-hop on a 64-bit bitset
-//                      adjacency, left filter and 2-hop candidates counted with a
-//                      per-wave counter array.   [reference: decompose,
-//                      src/device_funcs.cu:28-298]
+// Work unit: a job = (preprocessed graph in HBM, k, q, window of root
+// positions).  One plain run is one job; a batch or a server holds many jobs
+// whose roots share one pool of size-class launches.
+//   decompose_kernel   one wave64 per (job, root position), grid-stride:
+//                      later/earlier split of N(r), (q-2k)-core of the later
+//                      1-hop on a 64-bit bitset adjacency, left filter and
+//                      2-hop candidates counted with a per-wave counter array.
+//                      [reference: decompose, src/device_funcs.cu:28-298]
 //   build_kernel<WC>   one wave per valid root: bitset adjacency rows, the two
 //                      pair-class bitsets (class >= UNLINK2EQUAL, class >=
 //                      UNLINK2MORE), left rows, original ids.   [calculateDegrees,
@@ -13,9 +15,11 @@
 //   search_kernel<W>   persistent: every wave64 is a worker with a DFS stack of
 //                      (P, C, C2, X) bitset frames; k-search frames [kSearch
 //                      :1855-2444] and Branch frames [BNB :1516-1724] share the
-//                      stack; idle waves take roots from an atomic cursor or
-//                      frames donated by busy waves through a bounded MPMC queue
-//                      in HBM.
+//                      stack; idle waves take resumed frames, then new roots from
+//                      an atomic cursor, then frames donated by busy waves
+//                      through a bounded MPMC queue in HBM.  On a pause request
+//                      every wave writes its pending frames to a dump array and
+//                      exits; the host resumes them in a later launch.
 //
 // Wave layout: lane L of a wave owns the block vertices L, 64+L, ... (word j of
 // every bitset holds vertices 64j..64j+63, bit L = vertex 64j+L).  Sets are
@@ -24,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kp_device.h"
+#include "kp_engine.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -41,27 +46,49 @@ constexpr uint32_t CLAIM = 0x40000000u;
 constexpr uint32_t CMASK = 0x3FFFFFFFu;
 constexpr uint32_t SPIN_LIMIT = 1u << 24;
 
-struct RootInfo {          // decomposition result per root position
+struct RootInfo {          // decomposition result per (job, root position)
     int32_t status;        // 1 valid, 0 not, <0 error
     uint32_t n, h, l;
     uint64_t wr;           // CSR entries read: d(r) + sum over later neighbours, then + sum over B and L
 };
 
-struct RootDesc {          // one valid root in the subgraph arena
-    uint64_t off;          // u64-word offset in the arena
-    uint32_t n, h, l, pos;
+// One job's graph in HBM (local vertex ids 0..pn-1) and its parameters.
+struct JobDev {
+    const uint64_t *off;
+    const uint32_t *nbr, *deg, *orig;
+    const int32_t *dpos, *dseq;
+    uint32_t pn;
+    int32_t k, q;
+    uint32_t pad;
 };
 
+// Per-job result accumulators (sums mod 2^64), one 64-byte line each.
+struct JobAcc {
+    unsigned long long count, ha, hb, tasks, alg, pad[3];
+};
+static_assert(sizeof(JobAcc) == 64, "JobAcc is one line");
+
+struct RootDesc {          // one valid root; frames name roots by their index in the class table
+    uint64_t addr;         // device address of the root's subgraph
+    uint32_t n, h, l, slot;
+    uint32_t job;
+    uint16_t k, q;
+};
+static_assert(sizeof(RootDesc) == 32, "RootDesc layout");
+
 struct Sched {             // persistent-kernel scheduler state (one per launch)
-    uint32_t next_root;
+    uint32_t next_root;    // cursor over the start list
+    uint32_t next_seed;    // cursor over the resumed frames
     uint32_t active;
     uint32_t q_count;      // published frames over all sub-queues
     uint32_t hungry;       // waves currently looking for work
-    uint64_t q_head, q_tail;   // (unused; kept for layout)
-    int32_t error;
+    uint32_t pause;        // set once a pause is requested; waves dump and exit
     uint32_t exited;
-    uint64_t count, ha, hb, tasks, alg;
-    uint64_t list_used;
+    uint32_t dump_used;    // frames written to the dump array
+    int32_t error;
+    uint32_t pad0;
+    uint64_t epoch_frames; // frames processed, in steps of the check interval
+    uint64_t list_unused;
     uint64_t donated;
     uint64_t frames, spills, pops;
     uint64_t busy_ticks, idle_ticks;   // wall-clock ticks summed over waves
@@ -140,17 +167,15 @@ __device__ __forceinline__ void fence_wg() { __builtin_amdgcn_fence(__ATOMIC_SEQ
 
 // ============================================================ decompose
 struct DecompArgs {
-    const uint64_t *off;
-    const uint32_t *nbr, *deg;
-    const int32_t *dpos, *dseq;
-    uint32_t pn;
-    int k, q;
-    int64_t pos_begin, nroots;   // root positions [pos_begin, pos_begin + nroots)
+    const JobDev *jobs;
+    const uint32_t *slot_job, *slot_pos;   // per slot: job index and root position
+    uint32_t nslots;
     uint32_t *blk, *lft;
     uint32_t lcap;
     RootInfo *info;
     uint32_t *ws_cnt, *ws_b1c;
     uint64_t *ws_a1;
+    uint32_t cnt_stride;                   // counter words per wave (largest pn)
     uint32_t d1cap, w1cap, nwaves;
 };
 
@@ -169,18 +194,19 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
     const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
     const uint32_t gw = blockIdx.x * WPB + wv;
     if (gw >= a.nwaves) return;
-    uint32_t *cnt = a.ws_cnt + size_t(gw) * a.pn;
+    uint32_t *cnt = a.ws_cnt + size_t(gw) * a.cnt_stride;
     uint32_t *b1c = a.ws_b1c + size_t(gw) * a.d1cap;
     uint64_t *A1 = a.ws_a1 + size_t(gw) * a.d1cap * a.w1cap;
     uint64_t *row = s_row[wv], *alive = s_alive[wv];
     uint32_t *sb2 = s_b2[wv];
-    const int lb2k = a.q - 2 * a.k, bd = a.q - a.k;
 
-    for (int64_t slot = gw; slot < a.nroots; slot += a.nwaves) {
-        const int64_t pos = a.pos_begin + slot;
-        const uint32_t r = uint32_t(a.dseq[pos]);
-        const uint64_t ro = a.off[r];
-        const uint32_t dr = a.deg[r];
+    for (uint32_t slot = gw; slot < a.nslots; slot += a.nwaves) {
+        const JobDev J = a.jobs[rfl(a.slot_job[slot])];
+        const int lb2k = J.q - 2 * J.k, bd = J.q - J.k;
+        const int32_t pos = int32_t(rfl(a.slot_pos[slot]));
+        const uint32_t r = uint32_t(J.dseq[pos]);
+        const uint64_t ro = J.off[r];
+        const uint32_t dr = J.deg[r];
         uint32_t *B = a.blk + size_t(slot) * NCAP;
         uint32_t *L = a.lft + size_t(slot) * a.lcap;
         RootInfo info{0, 0, 0, 0, dr};
@@ -191,14 +217,14 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
         for (uint32_t base = 0; base < dr; base += 64) {
             const uint32_t idx = base + ln;
             const bool v = idx < dr;
-            const uint32_t w = v ? a.nbr[ro + idx] : 0u;
-            const bool later = v && a.dpos[w] > int32_t(pos);
+            const uint32_t w = v ? J.nbr[ro + idx] : 0u;
+            const bool later = v && J.dpos[w] > pos;
             const bool earlier = v && !later;
             const uint64_t mL = __ballot(later), mE = __ballot(earlier);
             if (later) {
                 const uint32_t t = m1 + __popcll(mL & lt_mask());
                 if (t < a.d1cap) b1c[t] = w;
-                wsum += a.deg[w];
+                wsum += J.deg[w];
             }
             if (earlier) {
                 const uint32_t t = ml + __popcll(mE & lt_mask());
@@ -224,15 +250,15 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
         fence_agent();
         for (uint32_t i = 0; i < m1; i++) {
             const uint32_t u = b1c[i];
-            const uint32_t du = a.deg[u];
+            const uint32_t du = J.deg[u];
             if (int(du) < lb2k) continue;          // cannot be in the core: row never read
-            const uint64_t uo = a.off[u];
+            const uint64_t uo = J.off[u];
             for (uint32_t t = ln; t < w1; t += 64) row[t] = 0;
             fence_wg();
             for (uint32_t base = 0; base < du; base += 64) {
                 const uint32_t idx = base + ln;
                 if (idx < du) {
-                    const uint32_t c = ald(&cnt[a.nbr[uo + idx]]);
+                    const uint32_t c = ald(&cnt[J.nbr[uo + idx]]);
                     if (c) {
                         const uint32_t j = c - 1;
                         atomicOr((unsigned long long *)&row[j >> 6], 1ull << (j & 63));
@@ -245,7 +271,7 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
         for (uint32_t t = ln; t < m1; t += 64) ast(&cnt[b1c[t]], 0);
         for (uint32_t t = 0; t < w1; t++) {
             const uint32_t i = 64 * t + ln;
-            const bool al = i < m1 && int(a.deg[b1c[i]]) >= lb2k;
+            const bool al = i < m1 && int(J.deg[b1c[i]]) >= lb2k;
             const uint64_t m = __ballot(al);
             if (ln == 0) alive[t] = m;
         }
@@ -295,12 +321,12 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
 
         // 4) mark r and N(r); 5) count |N(w) ∩ B1| for every w adjacent to B1
         if (ln == 0) aor(&cnt[r], EXCL);
-        for (uint32_t t = ln; t < dr; t += 64) aor(&cnt[a.nbr[ro + t]], EXCL);
+        for (uint32_t t = ln; t < dr; t += 64) aor(&cnt[J.nbr[ro + t]], EXCL);
         for (uint32_t b = 1; b < hop; b++) {
             const uint32_t u = B[b];
-            const uint64_t uo = a.off[u];
-            const uint32_t du = a.deg[u];
-            for (uint32_t t = ln; t < du; t += 64) aadd(&cnt[a.nbr[uo + t]], 1);
+            const uint64_t uo = J.off[u];
+            const uint32_t du = J.deg[u];
+            for (uint32_t t = ln; t < du; t += 64) aadd(&cnt[J.nbr[uo + t]], 1);
         }
         fence_agent();
 
@@ -321,17 +347,17 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
         uint32_t nb2 = 0, nl2 = 0;
         for (uint32_t b = 1; b < hop; b++) {
             const uint32_t u = B[b];
-            const uint64_t uo = a.off[u];
-            const uint32_t du = a.deg[u];
+            const uint64_t uo = J.off[u];
+            const uint32_t du = J.deg[u];
             for (uint32_t base = 0; base < du; base += 64) {
                 const uint32_t idx = base + ln;
                 bool own = false, later = false;
                 uint32_t w = 0;
                 if (idx < du) {
-                    w = a.nbr[uo + idx];
+                    w = J.nbr[uo + idx];
                     const uint32_t c = ald(&cnt[w]);
                     if (!(c & (EXCL | CLAIM))) {
-                        later = a.dpos[w] > int32_t(pos);
+                        later = J.dpos[w] > pos;
                         const int thr = lb2k + (later ? 2 : 3);
                         if (int(c & CMASK) >= thr) own = !(aor(&cnt[w], CLAIM) & CLAIM);
                     }
@@ -352,12 +378,12 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
         // 8) restore the counter array to zero
         fence_agent();
         if (ln == 0) ast(&cnt[r], 0);
-        for (uint32_t t = ln; t < dr; t += 64) ast(&cnt[a.nbr[ro + t]], 0);
+        for (uint32_t t = ln; t < dr; t += 64) ast(&cnt[J.nbr[ro + t]], 0);
         for (uint32_t b = 1; b < hop; b++) {
             const uint32_t u = B[b];
-            const uint64_t uo = a.off[u];
-            const uint32_t du = a.deg[u];
-            for (uint32_t t = ln; t < du; t += 64) ast(&cnt[a.nbr[uo + t]], 0);
+            const uint64_t uo = J.off[u];
+            const uint32_t du = J.deg[u];
+            for (uint32_t t = ln; t < du; t += 64) ast(&cnt[J.nbr[uo + t]], 0);
         }
         fence_agent();
         const uint32_t n = hop + nb2, l = nl1 + nl2;
@@ -366,7 +392,7 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
         info.l = l;
         if (n > NCAP) info.status = -1;
         else if (ml > a.lcap || l > a.lcap) info.status = -2;
-        else info.status = int(n) >= a.q ? 1 : 0;
+        else info.status = int(n) >= J.q ? 1 : 0;
         // 9) 2-hop later vertices in ascending id order (rank sort in LDS)
         if (info.status == 1 && nb2 > 1) {
             for (uint32_t t = ln; t < nb2; t += 64) sb2[t] = B[hop + t];
@@ -385,16 +411,13 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
 
 // ============================================================ build
 struct BuildArgs {
-    const uint64_t *off;
-    const uint32_t *nbr, *deg, *orig;
+    const JobDev *jobs;
     const uint32_t *blk, *lft;
     uint32_t lcap;
     const RootDesc *desc;
     uint32_t ndesc;
-    uint64_t *arena;
     uint32_t *ws_cnt;
-    uint32_t pn, nwaves;
-    int k, q;
+    uint32_t cnt_stride, nwaves;
     uint64_t *wr_extra;   // per desc: sum of degrees over B and L
 };
 
@@ -410,20 +433,21 @@ __global__ __launch_bounds__(256) void build_kernel(BuildArgs a)
     const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
     const uint32_t gw = blockIdx.x * WPB + wv;
     if (gw >= a.nwaves) return;
-    uint32_t *cnt = a.ws_cnt + size_t(gw) * a.pn;
+    uint32_t *cnt = a.ws_cnt + size_t(gw) * a.cnt_stride;
     uint64_t *row = s_row[wv];
-    const int k = a.k, q = a.q;
-    const int km2 = max(k - 2, 0), km3 = max(k - 3, 0);
-    const int thPP1 = q - k - 2 * km2, thPP2 = q - k - 2 * km3;
-    const int thPC1 = q - 2 * k - km2, thPC2 = q - k - 1 - km2 - km3;
-    const int thCC1 = q - 2 * k - (k - 1), thCC2 = q - 2 * k + 2 - (k - 1);
 
     for (uint32_t di = gw; di < a.ndesc; di += a.nwaves) {
         const RootDesc d = a.desc[di];
+        const JobDev J = a.jobs[d.job];
+        const int k = d.k, q = d.q;
+        const int km2 = max(k - 2, 0), km3 = max(k - 3, 0);
+        const int thPP1 = q - k - 2 * km2, thPP2 = q - k - 2 * km3;
+        const int thPC1 = q - 2 * k - km2, thPC2 = q - k - 1 - km2 - km3;
+        const int thCC1 = q - 2 * k - (k - 1), thCC2 = q - 2 * k + 2 - (k - 1);
         const uint32_t n = d.n, h = d.h, l = d.l;
-        const uint32_t *B = a.blk + size_t(d.pos) * NCAP;
-        const uint32_t *L = a.lft + size_t(d.pos) * a.lcap;
-        uint64_t *adj = a.arena + d.off;
+        const uint32_t *B = a.blk + size_t(d.slot) * NCAP;
+        const uint32_t *L = a.lft + size_t(d.slot) * a.lcap;
+        uint64_t *adj = reinterpret_cast<uint64_t *>(d.addr);
         uint64_t *c2 = adj + size_t(n) * WC;
         uint64_t *c4 = c2 + size_t(n) * WC;
         uint64_t *ladj = c4 + size_t(n) * WC;
@@ -435,14 +459,14 @@ __global__ __launch_bounds__(256) void build_kernel(BuildArgs a)
         uint64_t wsum = 0;
         for (uint32_t i = 0; i < n; i++) {
             const uint32_t u = B[i];
-            const uint64_t uo = a.off[u];
-            const uint32_t du = a.deg[u];
+            const uint64_t uo = J.off[u];
+            const uint32_t du = J.deg[u];
             if (ln < WC) row[ln] = 0;
             fence_wg();
             for (uint32_t base = 0; base < du; base += 64) {
                 const uint32_t idx = base + ln;
                 if (idx < du) {
-                    const uint32_t c = ald(&cnt[a.nbr[uo + idx]]);
+                    const uint32_t c = ald(&cnt[J.nbr[uo + idx]]);
                     if (c & CLAIM) {
                         const uint32_t j = (c & CMASK) - 1;
                         atomicOr((unsigned long long *)&ladj[size_t(j) * WC + (i >> 6)], 1ull << (i & 63));
@@ -456,8 +480,8 @@ __global__ __launch_bounds__(256) void build_kernel(BuildArgs a)
             if (ln < WC) adj[size_t(i) * WC + ln] = row[ln];
             if (ln == 0) wsum += du;
         }
-        for (uint32_t j = ln; j < l; j += 64) wsum += a.deg[L[j]];
-        for (uint32_t j = ln; j < n; j += 64) origp[j] = a.orig[B[j]];
+        for (uint32_t j = ln; j < l; j += 64) wsum += J.deg[L[j]];
+        for (uint32_t j = ln; j < n; j += 64) origp[j] = J.orig[B[j]];
         wsum = wsum64(wsum);
         if (ln == 0) a.wr_extra[di] = wsum;
         fence_agent();
@@ -520,23 +544,29 @@ __global__ __launch_bounds__(256) void build_kernel(BuildArgs a)
 
 // ============================================================ search
 struct SearchArgs {
-    const RootDesc *desc;
-    uint32_t ndesc;
-    const uint64_t *arena;
-    int k, q;
+    const RootDesc *desc;      // class root table
+    const uint32_t *todo;      // roots to start, in order (indices into desc)
+    uint32_t ntodo;
+    const uint64_t *seed;      // frames to resume
+    uint32_t nseed;
+    uint64_t *dump;            // frames written on pause
+    uint32_t dump_cap;
+    JobAcc *acc;               // per-job accumulators
     uint64_t *qframes, *qseq;  // NSUBQ * qcap slots
     uint32_t qcap;             // slots per shard
     SubQ *subq;
     Sched *sched;
     uint32_t *list;
     uint64_t list_cap;
-    uint64_t *list_used;     // shared by all size-class launches
+    uint64_t *list_used;       // shared by all size-class launches
     uint32_t nwaves;
-    uint32_t ring_cap;       // frames per wave in the LDS ring
-    uint32_t back_cap;       // frames per wave in the HBM backing stack
-    uint64_t *gstack;        // nwaves * back_cap frames
+    uint32_t ring_cap;         // frames per wave in the LDS ring
+    uint32_t back_cap;         // frames per wave in the HBM backing stack
+    uint64_t *gstack;          // nwaves * back_cap frames
     uint32_t donate_period, donate_target;
-    uint64_t limit_ticks;    // 0 = no limit; else wall-clock ticks per launch
+    uint64_t limit_ticks;      // 0 = no limit; else wall-clock ticks per launch
+    const uint32_t *host_pause;   // mapped host word (nonzero = pause) or null
+    uint64_t pause_frames;     // 0 = none; else pause once this many frames were processed
 };
 
 enum : uint32_t { FT_BRANCH = 0, FT_KSEARCH = 1 };
@@ -554,6 +584,8 @@ struct FrameLayout {
     static constexpr int FW = (4 * W + 2) & ~1;
 };
 
+constexpr int FrameWords(int W) { return (4 * W + 2) & ~1; }
+
 template <int W>
 __device__ __forceinline__ uint64_t rdl64(uint64_t x, uint32_t l)
 {
@@ -561,6 +593,15 @@ __device__ __forceinline__ uint64_t rdl64(uint64_t x, uint32_t l)
     const uint32_t hi = __builtin_amdgcn_readlane(int(uint32_t(x >> 32)), int(l));
     return (uint64_t(hi) << 32) | lo;
 }
+
+// Per-wave state that the frame loop touches rarely (root switches, checks,
+// idle transitions).  It lives in LDS so that it takes no scalar registers
+// from the frame loop; every lane writes the same (uniform) value.
+struct Cold {
+    uint64_t t_start, t_mark, busy_t, idle_t;
+    uint64_t alg, task0, bpt, frames, pops;
+    uint32_t cjob, check_every;
+};
 
 // LW: left-vertex rows held in registers (left vertices 64t+lane, t < LW)
 template <int W, int LW>
@@ -575,27 +616,38 @@ struct Worker {
     uint32_t rb = 0, rn = 0, bb = 0, bn = 0;
     // current root (uniform)
     uint32_t cur = 0xffffffffu, n = 0, h = 0, l = 0;
+    Cold *cold;
     const uint64_t *LA = nullptr;
     // rows of the vertices this lane owns (64j + lane), kept in VGPRs
     uint64_t adjR[W][W], c2R[W][W], c4R[W][W];
     uint64_t lftR[LW > 0 ? LW : 1][W];
     uint32_t origR[W];
-    // accumulators (uniform)
-    uint64_t cnt = 0, ha = 0, hb = 0, tasks = 0, alg = 0, frames = 0, spills = 0, pops = 0;
+    // accumulators (uniform); the bytes of the tasks are kept in Cold
+    uint64_t cnt = 0, ha = 0, hb = 0, tasks = 0;
+    uint32_t spills = 0;
     int err = 0;
-    uint64_t t_start = 0;
 
-    __device__ Worker(const SearchArgs &args, uint64_t *ring_, uint64_t *back_)
-        : a(args), S(args.sched), ring(ring_), back(back_)
+    __device__ Worker(const SearchArgs &args, uint64_t *ring_, uint64_t *back_, Cold *cold_)
+        : a(args), S(args.sched), ring(ring_), back(back_), cold(cold_)
     {
         ln = __lane_id();
-        k = a.k;
-        q = a.q;
-        t_start = wall_clock64();
+        k = 0;
+        q = 0;
+        const uint64_t now = wall_clock64();
+        cold->t_start = now;
+        cold->t_mark = now;
+        cold->busy_t = cold->idle_t = 0;
+        cold->alg = cold->task0 = cold->bpt = cold->frames = cold->pops = 0;
+        cold->cjob = 0xffffffffu;
+        // frames between checks of the error flag, the deadline and the pause
+        // sources; a frame budget shortens the interval so that small budgets
+        // are met about evenly by all waves
+        cold->check_every =
+            a.pause_frames ? uint32_t(max(uint64_t(32), min(uint64_t(1024), a.pause_frames / max(1u, a.nwaves)))) : 1024u;
     }
     __device__ __forceinline__ bool past_deadline() const
     {
-        return a.limit_ticks && wall_clock64() - t_start > a.limit_ticks;
+        return a.limit_ticks && wall_clock64() - cold->t_start > a.limit_ticks;
     }
     __device__ __forceinline__ void set_error(int code)
     {
@@ -809,8 +861,32 @@ struct Worker {
             atomicSub(&a.subq[shard].size, 1u);
             __hip_atomic_fetch_sub(&S->q_count, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
         }
-        pops++;
+        cold->pops = cold->pops + 1;
         return true;
+    }
+
+    // task bytes of the current root so far go to Cold::alg
+    __device__ void close_alg()
+    {
+        cold->alg = cold->alg + (tasks - cold->task0) * cold->bpt;
+        cold->task0 = tasks;
+    }
+    // adds the accumulators to their job's totals (close_alg first)
+    __device__ void flush()
+    {
+        const uint32_t job = cold->cjob;
+        const uint64_t alg = cold->alg;
+        if (job != 0xffffffffu && ln == 0 && (cnt | tasks)) {
+            JobAcc *J = a.acc + job;
+            atomicAdd(&J->count, (unsigned long long)cnt);
+            atomicAdd(&J->ha, (unsigned long long)ha);
+            atomicAdd(&J->hb, (unsigned long long)hb);
+            atomicAdd(&J->tasks, (unsigned long long)tasks);
+            atomicAdd(&J->alg, (unsigned long long)alg);
+        }
+        cnt = ha = hb = tasks = 0;
+        cold->alg = 0;
+        cold->task0 = 0;
     }
 
     // ---- root cache: this lane's rows of the current root ----
@@ -820,8 +896,17 @@ struct Worker {
         n = rfl(d->n);
         h = rfl(d->h);
         l = rfl(d->l);
-        const uint64_t off = rfl64(d->off);
-        const uint64_t *A = a.arena + off;
+        const uint32_t job = rfl(d->job);
+        const uint32_t kq = rfl(*reinterpret_cast<const uint32_t *>(&d->k));
+        k = int(kq & 0xffffu);
+        q = int(kq >> 16);
+        close_alg();
+        if (job != cold->cjob) {
+            flush();
+            cold->cjob = job;
+        }
+        cold->bpt = 2ull * (5ull * n + 16ull);
+        const uint64_t *A = reinterpret_cast<const uint64_t *>(rfl64(d->addr));
         const uint64_t *C2 = A + size_t(n) * W;
         const uint64_t *C4 = C2 + size_t(n) * W;
         LA = C4 + size_t(n) * W;
@@ -961,14 +1046,17 @@ struct Worker {
         hb += mix64(sb + uint64_t(s) * kp::HASH_LB);
         if (a.list) {
             uint64_t base = 0;
-            if (ln == 0) base = atomicAdd((unsigned long long *)a.list_used, (unsigned long long)(s + 1));
+            if (ln == 0) base = atomicAdd((unsigned long long *)a.list_used, (unsigned long long)(s + 2));
             base = rfl64(base);
-            if (base + s + 1 > a.list_cap) {
+            if (base + s + 2 > a.list_cap) {
                 set_error(KP_EOVERFLOW);
                 return;
             }
-            if (ln == 0) a.list[base] = s;
-            uint32_t o = uint32_t(base) + 1;
+            if (ln == 0) {
+                a.list[base] = s;
+                a.list[base + 1] = cold->cjob;
+            }
+            uint64_t o = base + 2;
 #pragma unroll
             for (int j = 0; j < W; j++) {
                 if (has(Sx, j)) a.list[o + __popcll(Sx[j] & lt_mask())] = origR[j];
@@ -977,7 +1065,7 @@ struct Worker {
         }
     }
 
-    __device__ __forceinline__ void count_task() { tasks++; alg += 2ull * (5ull * n + 16ull); }
+    __device__ __forceinline__ void count_task() { tasks++; }
 
     // ---- Branch frame (BNB :1516-1724) ----
     // The exclude child goes on the stack; the include child, when it exists,
@@ -1177,6 +1265,7 @@ struct Worker {
     {
         const RootDesc *d = a.desc + ridx;
         const uint32_t rn_ = rfl(d->n), rh = rfl(d->h);
+        const uint32_t rk = rfl(*reinterpret_cast<const uint32_t *>(&d->k)) & 0xffffu;
         uint64_t P[W], C[W], D[W], X[W];
 #pragma unroll
         for (int w = 0; w < W; w++) {
@@ -1192,22 +1281,66 @@ struct Worker {
             D[w] = range(int(rh), int(rn_));
             X[w] = 0;
         }
-        push(make_meta(ridx, FT_KSEARCH, uint32_t(k - 1)), P, C, X, D, true);
+        push(make_meta(ridx, FT_KSEARCH, rk - 1u), P, C, X, D, true);
+    }
+
+    // Pause sources: the scheduler flag, the host word, the frame budget.
+    // Returns true once a pause is in force.
+    __device__ bool pause_requested(uint32_t new_frames)
+    {
+        uint32_t pz = 0;
+        if (ln == 0) {
+            pz = __hip_atomic_load(&S->pause, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (!pz && a.host_pause &&
+                __hip_atomic_load(a.host_pause, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0)
+                pz = 1;
+            if (!pz && a.pause_frames && new_frames) {
+                const uint64_t old = atomicAdd((unsigned long long *)&S->epoch_frames, (unsigned long long)new_frames);
+                if (old + new_frames >= a.pause_frames) pz = 1;
+            }
+            if (pz) __hip_atomic_store(&S->pause, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return rfl(pz) != 0;
+    }
+
+    // Writes this wave's pending frames (the next frame, then the stack) to
+    // the dump array.
+    __device__ void dump_frames(bool have, const Frame<W> &f)
+    {
+        const uint32_t m = depth() + (have ? 1u : 0u);
+        if (m == 0) return;
+        uint32_t base = 0;
+        if (ln == 0) base = atomicAdd(&S->dump_used, m);
+        base = rfl(base);
+        if (uint64_t(base) + m > a.dump_cap) {
+            set_error(KP_EOVERFLOW);
+            return;
+        }
+        uint64_t *d = a.dump + size_t(base) * FW;
+        if (have) {
+            store_slot(d, make_meta(f.root, f.type, f.r), f.P, f.C, f.X, f.D, f.type == FT_KSEARCH);
+            d += FW;
+        }
+        for (uint32_t i = 0; i < bn; i++, d += FW) copy_slot(d, back_slot(i));
+        for (uint32_t i = 0; i < rn; i++, d += FW) copy_slot(d, ring_slot(i));
+        rn = bn = 0;
     }
 
     // Termination: a wave counts itself in S->active only while it holds
-    // frames, and it increments S->active before taking a frame or a root
-    // (seq_cst), so "no roots left, queue empty, nobody active", read in that
-    // order, means no frame exists anywhere.  A wave that reads it exits; a
-    // wave that still holds work never exits before its stack is empty.
-    // Waves without work count themselves in S->hungry; busy waves donate
-    // their oldest frames (the largest pending subtrees) while hungry waves
-    // outnumber the queued frames.
+    // frames, and it increments S->active before taking a frame, a seed or a
+    // root (seq_cst), so "no seeds or roots left, queue empty, nobody
+    // active", read in that order, means no frame exists anywhere.  A wave
+    // that reads it exits; a wave that still holds work never exits before
+    // its stack is empty, unless a pause is in force: then it dumps its
+    // frames and exits, and the last wave to exit moves the queued frames to
+    // the dump as well.  Waves without work count themselves in S->hungry;
+    // busy waves donate their oldest frames (the largest pending subtrees)
+    // while hungry waves outnumber the queued frames.
     __device__ void run()
     {
-        bool active_self = false, hungry_self = false, have = false;
-        uint32_t since_donate = 0, since_check = 0, backoff = 1;
-        uint64_t t_mark = t_start, busy_t = 0, idle_t = 0;
+        bool active_self = false, hungry_self = false, have = false, paused = false;
+        uint32_t since_donate = 0, backoff = 1;
+        uint32_t to_check = cold->check_every;   // frames left until the next check
         const uint32_t gw = blockIdx.x * 4u + (threadIdx.x >> 6);
         Frame<W> f, nf;
         for (;;) {
@@ -1215,19 +1348,22 @@ struct Worker {
                 if (depth() == 0) {
                     if (active_self) {
                         const uint64_t now = wall_clock64();
-                        busy_t += now - t_mark;
-                        t_mark = now;
+                        cold->busy_t = cold->busy_t + (now - cold->t_mark);
+                        cold->t_mark = now;
                         if (ln == 0) __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                         active_self = false;
                     }
                     if (err) break;
+                    // hints only (relaxed): claims below are atomic read-modify-writes
                     bool got = false;
-                    uint32_t qc = 0, nr = 0;
+                    uint32_t qc = 0, nr = 0, ns = 0;
                     if (ln == 0) {
-                        qc = __hip_atomic_load(&S->q_count, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                        nr = __hip_atomic_load(&S->next_root, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                        qc = __hip_atomic_load(&S->q_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ns = __hip_atomic_load(&S->next_seed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        nr = __hip_atomic_load(&S->next_root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
                     qc = rfl(qc);
+                    ns = rfl(ns);
                     nr = rfl(nr);
                     if (qc > 0) {
                         if (ln == 0) __hip_atomic_fetch_add(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
@@ -1240,18 +1376,34 @@ struct Worker {
                         }
                         if (err) break;
                     }
-                    if (!got && nr < a.ndesc) {
+                    if (!got && ns < a.nseed) {
+                        uint32_t si = 0xffffffffu;
+                        if (ln == 0) {
+                            __hip_atomic_fetch_add(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                            si = __hip_atomic_fetch_add(&S->next_seed, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                            if (si >= a.nseed)
+                                __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        si = rfl(si);
+                        if (si < a.nseed) {
+                            active_self = true;
+                            load_slot(a.seed + size_t(si) * FW, f);
+                            have = true;
+                            got = true;
+                        }
+                    }
+                    if (!got && nr < a.ntodo) {
                         uint32_t rr = 0xffffffffu;
                         if (ln == 0) {
                             __hip_atomic_fetch_add(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                             rr = __hip_atomic_fetch_add(&S->next_root, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                            if (rr >= a.ndesc)
+                            if (rr >= a.ntodo)
                                 __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                         }
                         rr = rfl(rr);
-                        if (rr < a.ndesc) {
+                        if (rr < a.ntodo) {
                             active_self = true;
-                            push_root(rr);
+                            push_root(a.todo[rr]);
                             got = true;
                         }
                     }
@@ -1262,10 +1414,18 @@ struct Worker {
                         }
                         uint32_t done = 0, stop = 0;
                         if (ln == 0) {
-                            const uint32_t nr2 = __hip_atomic_load(&S->next_root, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                            const uint32_t qc2 = __hip_atomic_load(&S->q_count, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                            const uint32_t ac = __hip_atomic_load(&S->active, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                            done = nr2 >= a.ndesc && qc2 == 0 && ac == 0;
+                            // a relaxed look first; the seq_cst reads (in this
+                            // order) confirm termination
+                            const bool maybe =
+                                __hip_atomic_load(&S->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+                                __hip_atomic_load(&S->q_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+                            if (maybe) {
+                                const uint32_t ns2 = __hip_atomic_load(&S->next_seed, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                                const uint32_t nr2 = __hip_atomic_load(&S->next_root, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                                const uint32_t qc2 = __hip_atomic_load(&S->q_count, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                                const uint32_t ac = __hip_atomic_load(&S->active, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                                done = ns2 >= a.nseed && nr2 >= a.ntodo && qc2 == 0 && ac == 0;
+                            }
                             stop = __hip_atomic_load(&S->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
                             if (past_deadline()) stop = 2;
                         }
@@ -1273,6 +1433,10 @@ struct Worker {
                         stop = rfl(stop);
                         if (stop == 2) set_error(KP_ETIMEDOUT);
                         if (done || stop) break;
+                        if (pause_requested(0)) {
+                            paused = true;
+                            break;
+                        }
                         // jittered exponential backoff: waves do not poll in lockstep
                         const uint32_t jit = (gw * 2654435761u + backoff * 97u) >> 28;
                         for (uint32_t z = 0; z < backoff + jit; z++) __builtin_amdgcn_s_sleep(16);
@@ -1285,12 +1449,11 @@ struct Worker {
                     }
                     backoff = 1;
                     const uint64_t now = wall_clock64();
-                    idle_t += now - t_mark;
-                    t_mark = now;
+                    cold->idle_t = cold->idle_t + (now - cold->t_mark);
+                    cold->t_mark = now;
                 }
                 if (!have) pop_top(f);
             }
-            frames++;
             if (f.root != cur) load_root(f.root);
             have = (f.type == FT_KSEARCH) ? ksearch(f, nf) : branch(f, nf);
             if (err) break;
@@ -1319,8 +1482,10 @@ struct Worker {
                     if (err) break;
                 }
             }
-            if (++since_check >= 1024) {
-                since_check = 0;
+            if (--to_check == 0) {
+                const uint32_t ce = cold->check_every;
+                to_check = ce;
+                cold->frames = cold->frames + ce;
                 uint32_t stop = 0;
                 if (ln == 0) {
                     stop = __hip_atomic_load(&S->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
@@ -1329,42 +1494,66 @@ struct Worker {
                 stop = rfl(stop);
                 if (stop == 2) set_error(KP_ETIMEDOUT);
                 if (stop) break;
+                if (pause_requested(ce)) {
+                    paused = true;
+                    break;
+                }
             }
         }
+        if (paused && !err) dump_frames(have, f);
         if (active_self && ln == 0) __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
         if (hungry_self && ln == 0) atomicSub(&S->hungry, 1u);
         {
             const uint64_t now = wall_clock64();
-            if (active_self) busy_t += now - t_mark;
-            else idle_t += now - t_mark;
+            if (active_self) cold->busy_t = cold->busy_t + (now - cold->t_mark);
+            else cold->idle_t = cold->idle_t + (now - cold->t_mark);
         }
+        close_alg();
+        flush();
+        const uint64_t busy_t = cold->busy_t, idle_t = cold->idle_t, pops = cold->pops;
+        const uint64_t frames = cold->frames + (cold->check_every - to_check);
+        uint32_t last = 0;
         if (ln == 0) {
             atomicAdd((unsigned long long *)&S->busy_ticks, (unsigned long long)busy_t);
             atomicAdd((unsigned long long *)&S->idle_ticks, (unsigned long long)idle_t);
-            atomicAdd((unsigned long long *)&S->count, (unsigned long long)cnt);
-            atomicAdd((unsigned long long *)&S->ha, (unsigned long long)ha);
-            atomicAdd((unsigned long long *)&S->hb, (unsigned long long)hb);
-            atomicAdd((unsigned long long *)&S->tasks, (unsigned long long)tasks);
-            atomicAdd((unsigned long long *)&S->alg, (unsigned long long)alg);
             atomicAdd((unsigned long long *)&S->frames, (unsigned long long)frames);
             atomicAdd((unsigned long long *)&S->spills, (unsigned long long)spills);
             atomicAdd((unsigned long long *)&S->pops, (unsigned long long)pops);
-            atomicAdd(&S->exited, 1u);
+            last = __hip_atomic_fetch_add(&S->exited, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT) + 1 == a.nwaves;
+        }
+        // The last wave out moves frames still queued to the dump: every other
+        // wave has exited, so nothing publishes or pops concurrently.
+        if (rfl(last)) {
+            uint32_t pz = 0, qc = 0;
+            if (ln == 0) {
+                pz = __hip_atomic_load(&S->pause, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                qc = __hip_atomic_load(&S->q_count, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (rfl(pz) && !err) {
+                for (uint32_t i = rfl(qc); i > 0 && !err; i--) {
+                    if (!q_pop(f)) break;
+                    dump_frames(true, f);
+                }
+            }
         }
     }
 };
 
+// The one-word class runs at 4 waves per SIMD (<= 128 VGPRs, no scratch).
+// The two-word class stays at its natural 2: capped at 3 it fits 168 VGPRs
+// without scratch but measured slower (soc-epinions 470 ms against 355 ms).
 template <int W, int LW>
-__global__ __launch_bounds__(256) void search_kernel(SearchArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1 ? 4 : 1))) void search_kernel(SearchArgs a)
 {
     extern __shared__ uint64_t lds[];
+    __shared__ Cold s_cold[4];
     constexpr int FW = FrameLayout<W>::FW;
     const int wv = threadIdx.x >> 6;
     const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + wv;
     if (gw >= a.nwaves) return;
     uint64_t *ring = lds + size_t(wv) * a.ring_cap * FW;
     uint64_t *back = a.gstack ? a.gstack + size_t(gw) * a.back_cap * FW : nullptr;
-    Worker<W, LW> wk(a, ring, back);
+    Worker<W, LW> wk(a, ring, back, &s_cold[wv]);
     wk.run();
 }
 
@@ -1377,8 +1566,9 @@ __global__ void iota_kernel(uint64_t *seq, uint32_t n, uint32_t per_shard)
 
 } // namespace kpd
 
+
 // ======================================================================
-// host driver
+// host side: the engine (kp_engine.h), prepared graphs, plain and batched runs
 // ======================================================================
 using namespace kpd;
 
@@ -1391,37 +1581,9 @@ using namespace kpd;
         }                                                                              \
     } while (0)
 
-namespace {
+namespace kp {
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    int ensure(size_t need)
-    {
-        if (need <= bytes && p) return KP_OK;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        if (need == 0) need = 8;
-        if (hipMalloc(&p, need) != hipSuccess) {
-            kp::set_error("hipMalloc(%zu) failed", need);
-            p = nullptr;
-            return KP_ENOMEM;
-        }
-        bytes = need;
-        return KP_OK;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
-
-constexpr int kClassW[] = {1, 2, 3, 4, 6, 8, 16};
-constexpr int kNumClass = 7;
+const int kClassW[kNumClass] = {1, 2, 3, 4, 6, 8, 16};
 
 int class_of(uint32_t n)
 {
@@ -1431,41 +1593,699 @@ int class_of(uint32_t n)
     return -1;
 }
 
+int DevBuf::ensure(size_t need)
+{
+    if (need <= bytes && p) return KP_OK;
+    release();
+    if (need == 0) need = 8;
+    if (hipMalloc(&p, need) != hipSuccess) {
+        set_error("hipMalloc(%zu) failed", need);
+        p = nullptr;
+        return KP_ENOMEM;
+    }
+    bytes = need;
+    return KP_OK;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+namespace {
+
+size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// the reference's task-queue capacity MAX_CAP (queue A), the
+// default for kp_params.queue_capacity
+constexpr uint64_t kDefaultQueueCap = 4194304;
+
 } // namespace
 
+size_t packed_graph_bytes(const Prep &p)
+{
+    const size_t n = p.g.n, m = p.g.m;
+    return al256(8 * (n + 1)) + al256(4 * m) + 4 * al256(4 * n);
+}
+
+int upload_graphs(const std::vector<const Prep *> &preps, DevBuf &buf, std::vector<GraphView> &views, hipStream_t st)
+{
+    std::vector<size_t> base(preps.size());
+    size_t total = 0;
+    for (size_t i = 0; i < preps.size(); i++) {
+        base[i] = total;
+        total += packed_graph_bytes(*preps[i]);
+    }
+    int rc = buf.ensure(std::max<size_t>(total, 256));
+    if (rc) return rc;
+    std::vector<uint8_t> host(total);
+    views.assign(preps.size(), GraphView{});
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t i = 0; i < preps.size(); i++) {
+        const Prep &P = *preps[i];
+        const kp_graph &G = P.g;
+        const size_t n = G.n, m = G.m;
+        uint8_t *h = host.data() + base[i];
+        const char *d = static_cast<const char *>(buf.p) + base[i];
+        size_t o = 0;
+        auto put = [&](const void *src, size_t bytes) -> const char * {
+            const size_t at = o;
+            if (bytes && src) std::memcpy(h + o, src, bytes);
+            else if (bytes) std::memset(h + o, 0, bytes);
+            o += al256(bytes);
+            return d + at;
+        };
+        GraphView &v = views[i];
+        v.off = reinterpret_cast<const uint64_t *>(put(G.off, 8 * (n + 1)));
+        v.nbr = reinterpret_cast<const uint32_t *>(put(G.nbr, 4 * m));
+        v.deg = reinterpret_cast<const uint32_t *>(put(G.deg, 4 * n));
+        v.dpos = reinterpret_cast<const int32_t *>(put(P.dpos.data(), 4 * n));
+        v.dseq = reinterpret_cast<const int32_t *>(put(P.dseq.data(), 4 * n));
+        v.orig = reinterpret_cast<const uint32_t *>(put(P.orig.data(), 4 * n));
+        v.pn = uint32_t(n);
+        v.pm = m;
+        uint32_t maxdeg = 0, d1 = 0;
+        for (uint32_t u = 0; u < G.n; u++) {
+            maxdeg = std::max(maxdeg, G.deg[u]);
+            uint32_t later = 0;
+            for (uint64_t e = G.off[u]; e < G.off[u + 1]; e++) later += P.dpos[G.nbr[e]] > P.dpos[u];
+            d1 = std::max(d1, later);
+        }
+        v.maxdeg = maxdeg;
+        v.d1max = d1;
+    }
+    if (total) HIPCHK(hipMemcpyAsync(buf.p, host.data(), total, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return KP_OK;
+}
+
+ClassState::~ClassState()
+{
+    if (spill) (void)hipHostFree(spill);
+    if (done_ev) (void)hipEventDestroy(done_ev);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Engine::init()
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        set_error("no HIP device visible: the HIP backend is required");
+        return KP_ENODEV;
+    }
+    HIPCHK(hipGetDevice(&device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    cus = prop.multiProcessorCount;
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    for (auto &c : cls) {
+        HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c.done_ev, hipEventDisableTiming));
+    }
+    return KP_OK;
+}
+
+Engine::~Engine()
+{
+    for (DevBuf *b : segs) delete b;
+    if (hpause) (void)hipHostFree(hpause);
+    if (side) (void)hipStreamDestroy(side);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+uint64_t Engine::device_bytes() const
+{
+    const DevBuf *all[] = {&jobs, &acc, &blk, &lft, &info, &slot_job, &slot_pos, &ws_cnt, &ws_b1c, &ws_a1,
+                           &bdesc, &wr_extra, &list, &list_used};
+    uint64_t s = 0;
+    for (const DevBuf *b : all) s += b->bytes;
+    for (const DevBuf *b : segs) s += b->bytes;
+    for (const auto &c : cls) {
+        const DevBuf *cb[] = {&c.q, &c.qseq, &c.sched, &c.stack, &c.subq, &c.desc, &c.todo, &c.seed[0], &c.seed[1], &c.dump};
+        for (const DevBuf *b : cb) s += b->bytes;
+    }
+    return s;
+}
+
+int Engine::reserve_jobs(size_t n)
+{
+    n = std::max<size_t>(n, 1);
+    if (hjobs.size() < n) hjobs.resize(n, JobDev{});
+    int rc;
+    if ((rc = jobs.ensure(sizeof(JobDev) * hjobs.size())) || (rc = acc.ensure(sizeof(JobAcc) * hjobs.size()))) return rc;
+    return KP_OK;
+}
+
+void Engine::set_job(uint32_t id, const GraphView &g, int k, int q)
+{
+    hjobs[id] = JobDev{g.off, g.nbr, g.deg, g.orig, g.dpos, g.dseq, g.pn, k, q, 0};
+}
+
+int Engine::sync_jobs()
+{
+    HIPCHK(hipMemcpyAsync(jobs.p, hjobs.data(), sizeof(JobDev) * hjobs.size(), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return KP_OK;
+}
+
+int Engine::zero_acc(uint32_t first, uint32_t count)
+{
+    if (count) HIPCHK(hipMemsetAsync(acc.as<JobAcc>() + first, 0, sizeof(JobAcc) * count, stream));
+    return KP_OK;
+}
+
+int Engine::read_acc(uint32_t first, uint32_t count, std::vector<uint64_t> &out5)
+{
+    std::vector<JobAcc> h(count);
+    out5.assign(size_t(count) * 5, 0);
+    if (!count) return KP_OK;
+    HIPCHK(hipMemcpyAsync(h.data(), acc.as<JobAcc>() + first, sizeof(JobAcc) * count, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (uint32_t i = 0; i < count; i++) {
+        out5[5 * i + 0] = h[i].count;
+        out5[5 * i + 1] = h[i].ha;
+        out5[5 * i + 2] = h[i].hb;
+        out5[5 * i + 3] = h[i].tasks;
+        out5[5 * i + 4] = h[i].alg;
+    }
+    return KP_OK;
+}
+
+bool Engine::has_work() const
+{
+    for (const auto &c : cls)
+        if (c.has_work()) return true;
+    return false;
+}
+
+void Engine::reset_tables()
+{
+    for (auto &c : cls) {
+        c.hdesc.clear();
+        c.htodo.clear();
+        c.todo_done = 0;
+        c.desc_uploaded = 0;
+        c.maxn = 0;
+        c.nseed = 0;
+        c.spill_n = 0;
+        c.launched = false;
+    }
+    for (DevBuf *b : segs) delete b;
+    segs.clear();
+}
+
+namespace {
+
+template <int WC>
+int launch_build(hipStream_t st, const BuildArgs &ba, uint32_t nwg)
+{
+    hipLaunchKernelGGL((build_kernel<WC>), dim3(nwg), dim3(256), 0, st, ba);
+    HIPCHK(hipGetLastError());
+    return KP_OK;
+}
+
+int run_build_class(hipStream_t st, int c, const BuildArgs &ba, uint32_t nwg)
+{
+    switch (kClassW[c]) {
+    case 1: return launch_build<1>(st, ba, nwg);
+    case 2: return launch_build<2>(st, ba, nwg);
+    case 3: return launch_build<3>(st, ba, nwg);
+    case 4: return launch_build<4>(st, ba, nwg);
+    case 6: return launch_build<6>(st, ba, nwg);
+    case 8: return launch_build<8>(st, ba, nwg);
+    case 16: return launch_build<16>(st, ba, nwg);
+    }
+    return KP_EINVAL;
+}
+
+} // namespace
+
+int Engine::stage(const std::vector<JobIn> &jv, std::vector<JobStats> &st,
+                  const std::function<uint64_t *(size_t i, uint64_t words)> &alloc)
+{
+    st.assign(jv.size(), JobStats{});
+    size_t S = 0;
+    uint32_t lcap = 1, d1cap = 1, stride = 1;
+    for (const JobIn &j : jv) {
+        S += size_t(std::max<int64_t>(0, j.pos_end - j.pos_begin));
+        const uint32_t pn1 = std::max<uint32_t>(j.g.pn, 1);
+        lcap = std::max(lcap, std::min<uint32_t>(pn1, std::max<uint32_t>(2048, 2 * j.g.maxdeg + 1024)));
+        d1cap = std::max(d1cap, j.g.d1max);
+        stride = std::max(stride, pn1);
+    }
+    if (S == 0) return KP_OK;
+    const uint32_t w1cap = (d1cap + 63) / 64;
+    std::vector<uint32_t> slot_i, slot_id, slot_p;
+    slot_i.reserve(S);
+    slot_id.reserve(S);
+    slot_p.reserve(S);
+    for (size_t i = 0; i < jv.size(); i++)
+        for (int64_t pos = jv[i].pos_begin; pos < jv[i].pos_end; pos++) {
+            slot_i.push_back(uint32_t(i));
+            slot_id.push_back(jv[i].id);
+            slot_p.push_back(uint32_t(pos));
+        }
+    const size_t per_slot = size_t(NCAP + lcap) * 4;
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>({S, 65536, (size_t(2) << 30) / per_slot}));
+    uint32_t nwaves = uint32_t(std::min<size_t>(chunk, size_t(cus) * 16));
+    nwaves = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(nwaves, (8ull << 30) / (uint64_t(stride) * 4))));
+    int rc;
+    const size_t cnt_bytes = size_t(nwaves) * stride * 4;
+    if ((rc = blk.ensure(chunk * NCAP * 4)) || (rc = lft.ensure(chunk * lcap * 4)) ||
+        (rc = info.ensure(chunk * sizeof(RootInfo))) || (rc = slot_job.ensure(chunk * 4)) ||
+        (rc = slot_pos.ensure(chunk * 4)) || (rc = ws_b1c.ensure(size_t(nwaves) * d1cap * 4)) ||
+        (rc = ws_a1.ensure(size_t(nwaves) * d1cap * w1cap * 8)))
+        return rc;
+    {
+        void *before = ws_cnt.p;
+        if ((rc = ws_cnt.ensure(cnt_bytes))) return rc;
+        if (ws_cnt.p != before || cnt_bytes > ws_cnt_zeroed) {
+            HIPCHK(hipMemsetAsync(ws_cnt.p, 0, ws_cnt.bytes, stream));
+            ws_cnt_zeroed = ws_cnt.bytes;
+        }
+    }
+    std::vector<RootInfo> hinfo(chunk);
+    std::vector<RootDesc> newd[kNumClass];
+    struct Pend {
+        int c;
+        size_t i;
+        uint64_t words;
+    };
+    for (size_t c0 = 0; c0 < S; c0 += chunk) {
+        const size_t cn = std::min(chunk, S - c0);
+        HIPCHK(hipMemcpyAsync(slot_job.p, slot_id.data() + c0, 4 * cn, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(slot_pos.p, slot_p.data() + c0, 4 * cn, hipMemcpyHostToDevice, stream));
+        const uint32_t nw = uint32_t(std::min<size_t>(nwaves, cn));
+        DecompArgs da{};
+        da.jobs = jobs.as<JobDev>();
+        da.slot_job = slot_job.as<uint32_t>();
+        da.slot_pos = slot_pos.as<uint32_t>();
+        da.nslots = uint32_t(cn);
+        da.blk = blk.as<uint32_t>();
+        da.lft = lft.as<uint32_t>();
+        da.lcap = lcap;
+        da.info = info.as<RootInfo>();
+        da.ws_cnt = ws_cnt.as<uint32_t>();
+        da.ws_b1c = ws_b1c.as<uint32_t>();
+        da.ws_a1 = ws_a1.as<uint64_t>();
+        da.cnt_stride = stride;
+        da.d1cap = d1cap;
+        da.w1cap = w1cap;
+        da.nwaves = nw;
+        hipLaunchKernelGGL(decompose_kernel, dim3((nw + WPB - 1) / WPB), dim3(256), 0, stream, da);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(hinfo.data(), info.p, sizeof(RootInfo) * cn, hipMemcpyDeviceToHost, stream));
+        if (hipStreamSynchronize(stream) != hipSuccess) {
+            ws_cnt_zeroed = 0;
+            set_error("decompose kernel failed");
+            return KP_EHIP;
+        }
+        // classes, per-job arena sizes
+        std::vector<RootDesc> cd[kNumClass];
+        std::vector<Pend> pend[kNumClass];
+        std::vector<uint64_t> jwords(jv.size(), 0);
+        for (size_t s = 0; s < cn; s++) {
+            const RootInfo &ri = hinfo[s];
+            const size_t i = slot_i[c0 + s];
+            if (ri.status < 0) {
+                if (ri.status == -1) {
+                    set_error("job %zu, root at degeneracy position %u: block exceeds %u vertices", i, slot_p[c0 + s], NCAP);
+                    return KP_ERANGE;
+                }
+                set_error("job %zu, root at degeneracy position %u: left set exceeds %u vertices", i, slot_p[c0 + s], lcap);
+                return KP_EOVERFLOW;
+            }
+            if (ri.status != 1) continue;
+            JobStats &js = st[i];
+            js.valid_roots++;
+            js.max_n = std::max(js.max_n, ri.n);
+            js.max_l = std::max(js.max_l, ri.l);
+            const int c = class_of(ri.n);
+            const uint64_t W = uint64_t(kClassW[c]);
+            const uint64_t words = 3ull * ri.n * W + uint64_t(ri.l) * W + (ri.n + 1) / 2;
+            RootDesc d{};
+            d.n = ri.n;
+            d.h = ri.h;
+            d.l = ri.l;
+            d.slot = uint32_t(s);
+            d.job = jv[i].id;
+            d.k = uint16_t(jv[i].k);
+            d.q = uint16_t(jv[i].q);
+            cd[c].push_back(d);
+            pend[c].push_back({c, i, words});
+            jwords[i] += words;
+        }
+        std::vector<uint64_t *> jbase(jv.size(), nullptr), jcur(jv.size(), nullptr);
+        for (size_t i = 0; i < jv.size(); i++)
+            if (jwords[i]) {
+                jbase[i] = alloc(i, jwords[i]);
+                if (!jbase[i]) {
+                    set_error("arena allocation of %llu bytes failed", (unsigned long long)(8 * jwords[i]));
+                    return KP_ENOMEM;
+                }
+                jcur[i] = jbase[i];
+                st[i].arena_bytes += 8 * jwords[i];
+            }
+        std::vector<RootDesc> all;
+        std::vector<uint32_t> cbeg(kNumClass + 1, 0);
+        for (int c = 0; c < kNumClass; c++) {
+            cbeg[c] = uint32_t(all.size());
+            for (size_t t = 0; t < cd[c].size(); t++) {
+                RootDesc d = cd[c][t];
+                d.addr = reinterpret_cast<uint64_t>(jcur[pend[c][t].i]);
+                jcur[pend[c][t].i] += pend[c][t].words;
+                all.push_back(d);
+            }
+        }
+        cbeg[kNumClass] = uint32_t(all.size());
+        const size_t nd = all.size();
+        std::vector<uint64_t> wr(nd);
+        if (nd) {
+            if ((rc = bdesc.ensure(sizeof(RootDesc) * nd)) || (rc = wr_extra.ensure(8 * nd))) return rc;
+            HIPCHK(hipMemcpyAsync(bdesc.p, all.data(), sizeof(RootDesc) * nd, hipMemcpyHostToDevice, stream));
+            for (int c = 0; c < kNumClass; c++) {
+                const uint32_t b = cbeg[c], e = cbeg[c + 1];
+                if (b == e) continue;
+                BuildArgs ba{};
+                ba.jobs = jobs.as<JobDev>();
+                ba.blk = blk.as<uint32_t>();
+                ba.lft = lft.as<uint32_t>();
+                ba.lcap = lcap;
+                ba.desc = bdesc.as<RootDesc>() + b;
+                ba.ndesc = e - b;
+                ba.ws_cnt = ws_cnt.as<uint32_t>();
+                ba.cnt_stride = stride;
+                ba.nwaves = std::min<uint32_t>(nw, ((e - b) + WPB - 1) / WPB * WPB);
+                ba.wr_extra = wr_extra.as<uint64_t>() + b;
+                if ((rc = run_build_class(stream, c, ba, (ba.nwaves + WPB - 1) / WPB))) return rc;
+            }
+            HIPCHK(hipMemcpyAsync(wr.data(), wr_extra.p, 8 * nd, hipMemcpyDeviceToHost, stream));
+            if (hipStreamSynchronize(stream) != hipSuccess) {
+                ws_cnt_zeroed = 0;
+                set_error("build kernel failed");
+                return KP_EHIP;
+            }
+        }
+        for (int c = 0; c < kNumClass; c++)
+            for (uint32_t t = cbeg[c]; t < cbeg[c + 1]; t++) {
+                const RootDesc &d = all[t];
+                const RootInfo &ri = hinfo[d.slot];
+                JobStats &js = st[pend[c][t - cbeg[c]].i];
+                const uint64_t w = ri.wr + wr[t];
+                js.adj_bytes += 4 * w;
+                js.alg_bytes += 4 * w + (uint64_t(ri.n) * ri.n + 7) / 8 + uint64_t(ri.n) * ri.n;
+                newd[c].push_back(d);
+            }
+    }
+    // largest blocks first within what this call adds (longest-processing-time order)
+    for (int c = 0; c < kNumClass; c++) {
+        auto &v = newd[c];
+        if (v.empty()) continue;
+        std::stable_sort(v.begin(), v.end(), [](const RootDesc &x, const RootDesc &y) { return x.n > y.n; });
+        ClassState &C = cls[c];
+        for (const RootDesc &d : v) {
+            C.htodo.push_back(uint32_t(C.hdesc.size()));
+            C.hdesc.push_back(d);
+            C.maxn = std::max(C.maxn, d.n);
+        }
+    }
+    return KP_OK;
+}
+
+namespace {
+
+template <int W, int LW>
+int launch_class(Engine &E, int c, const SearchOpts &o)
+{
+    ClassState &C = E.cls[c];
+    constexpr int FW = FrameLayout<W>::FW;
+    const uint32_t need = C.maxn + 3;   // DFS depth bound: every child shrinks C2 or C
+    const uint32_t ring = std::min<uint32_t>(need, std::max<uint32_t>(16, 8192 / (FW * 8)));
+    const uint32_t back = need > ring ? need - ring + 1 : 0;
+    const size_t dyn = size_t(ring) * FW * 8 * 4;
+    auto kern = search_kernel<W, LW>;
+    if (dyn > 64 * 1024) HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn)));
+    int nb = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, dyn));
+    if (nb < 1) nb = 1;
+    const uint32_t nwg = uint32_t(nb) * uint32_t(E.cus);
+    SearchArgs sa{};
+    sa.nwaves = nwg * 4;
+    sa.qcap = std::max<uint32_t>(256, sa.nwaves * 4 / NSUBQ);   // slots per shard
+    sa.ring_cap = ring;
+    sa.back_cap = back;
+    sa.donate_period = 8;
+    sa.donate_target = std::max<uint32_t>(16, sa.nwaves / 4);
+    const size_t nslots = size_t(sa.qcap) * NSUBQ;
+    int rc;
+    if ((rc = C.q.ensure(nslots * FW * 8)) || (rc = C.qseq.ensure(nslots * 8)) || (rc = C.sched.ensure(sizeof(Sched))) ||
+        (rc = C.subq.ensure(sizeof(SubQ) * NSUBQ)))
+        return rc;
+    if (back && (rc = C.stack.ensure(size_t(sa.nwaves) * back * FW * 8))) return rc;
+    uint32_t dump_cap = 0;
+    if (o.pausable) {
+        dump_cap = uint32_t(std::min<uint64_t>(0xffffffffu, uint64_t(sa.nwaves) * (need + 2) + nslots));
+        if ((rc = C.dump.ensure(size_t(dump_cap) * FW * 8))) return rc;
+    }
+    // root table (grown by whole re-upload) and the start list
+    if (C.hdesc.size() * sizeof(RootDesc) > C.desc.bytes) {
+        if ((rc = C.desc.ensure(2 * C.hdesc.size() * sizeof(RootDesc)))) return rc;
+        C.desc_uploaded = 0;
+    }
+    if (C.desc_uploaded < C.hdesc.size()) {
+        HIPCHK(hipMemcpyAsync(C.desc.as<RootDesc>() + C.desc_uploaded, C.hdesc.data() + C.desc_uploaded,
+                              sizeof(RootDesc) * (C.hdesc.size() - C.desc_uploaded), hipMemcpyHostToDevice, C.stream));
+        C.desc_uploaded = C.hdesc.size();
+    }
+    const size_t ntodo = C.htodo.size() - C.todo_done;
+    if ((rc = C.todo.ensure(4 * std::max<size_t>(ntodo, 1)))) return rc;
+    if (ntodo)
+        HIPCHK(hipMemcpyAsync(C.todo.p, C.htodo.data() + C.todo_done, 4 * ntodo, hipMemcpyHostToDevice, C.stream));
+    sa.desc = C.desc.as<RootDesc>();
+    sa.todo = C.todo.as<uint32_t>();
+    sa.ntodo = uint32_t(ntodo);
+    sa.seed = C.nseed ? C.seed[C.cur_seed].as<uint64_t>() : nullptr;
+    sa.nseed = C.nseed;
+    sa.dump = o.pausable ? C.dump.as<uint64_t>() : nullptr;
+    sa.dump_cap = dump_cap;
+    sa.acc = E.acc.as<JobAcc>();
+    sa.qframes = C.q.as<uint64_t>();
+    sa.qseq = C.qseq.as<uint64_t>();
+    sa.sched = C.sched.as<Sched>();
+    sa.subq = C.subq.as<SubQ>();
+    sa.gstack = back ? C.stack.as<uint64_t>() : nullptr;
+    sa.list = o.want_list ? E.list.as<uint32_t>() : nullptr;
+    sa.list_cap = o.list_cap;
+    sa.list_used = E.list_used.as<uint64_t>();
+    sa.limit_ticks = o.limit_ticks;
+    sa.host_pause = o.host_pause ? E.dpause : nullptr;
+    sa.pause_frames = o.pause_frames;
+    HIPCHK(hipMemsetAsync(sa.sched, 0, sizeof(Sched), C.stream));
+    HIPCHK(hipMemsetAsync(sa.subq, 0, sizeof(SubQ) * NSUBQ, C.stream));
+    hipLaunchKernelGGL(iota_kernel, dim3(uint32_t((nslots + 255) / 256)), dim3(256), 0, C.stream, sa.qseq,
+                       uint32_t(nslots), sa.qcap);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(kern, dim3(nwg), dim3(256), dyn, C.stream, sa);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(C.done_ev, C.stream));
+    C.launched = true;
+    C.nwaves = sa.nwaves;
+    C.ring = ring;
+    C.back = back;
+    C.dump_cap = dump_cap;
+    C.launch_todo = uint32_t(ntodo);
+    C.blocks_per_cu = nb;
+    return KP_OK;
+}
+
+int launch_class_w(Engine &E, int c, const SearchOpts &o)
+{
+    switch (kClassW[c]) {
+    case 1: return launch_class<1, 2>(E, c, o);
+    case 2: return launch_class<2, 2>(E, c, o);
+    case 3: return launch_class<3, 2>(E, c, o);
+    case 4: return launch_class<4, 1>(E, c, o);
+    case 6: return launch_class<6, 0>(E, c, o);
+    case 8: return launch_class<8, 0>(E, c, o);
+    case 16: return launch_class<16, 0>(E, c, o);
+    }
+    return KP_EINVAL;
+}
+
+int grow_spill(ClassState &C, size_t need, size_t fbytes)
+{
+    if (need <= C.spill_cap) return KP_OK;
+    const size_t cap = std::max<size_t>(need, 2 * C.spill_cap);
+    void *p = nullptr;
+    if (hipHostMalloc(&p, cap * fbytes, hipHostMallocDefault) != hipSuccess) {
+        set_error("pinned host allocation of %zu bytes failed", cap * fbytes);
+        return KP_ENOMEM;
+    }
+    if (C.spill_n) std::memcpy(p, C.spill, C.spill_n * fbytes);
+    if (C.spill) (void)hipHostFree(C.spill);
+    C.spill = static_cast<uint64_t *>(p);
+    C.spill_cap = cap;
+    return KP_OK;
+}
+
+// Pending frames after a launch: `D` dumped frames plus the resumed frames
+// [ns, nseed) it did not take.  Up to `budget` of them become the next
+// launch's resumed frames on the device; the rest are spilled to pinned host
+// memory (a stack), from which later launches refill.  All copies run on the
+// side stream.
+int carry_frames(Engine &E, ClassState &C, size_t fbytes, uint32_t D, uint32_t ns, uint64_t budget, EpochStats &es)
+{
+    const size_t R = size_t(std::max<uint64_t>(1, std::min<uint64_t>(budget, 0x7fffffffu)));
+    const uint32_t U = C.nseed > ns ? C.nseed - ns : 0;
+    const size_t want = std::min<size_t>(R, size_t(D) + U + C.spill_n);
+    DevBuf &nb = C.seed[1 - C.cur_seed];
+    int rc;
+    if (want && (rc = nb.ensure(std::max<size_t>(want, 1) * fbytes))) return rc;
+    char *dst = static_cast<char *>(nb.p);
+    const char *dump = static_cast<const char *>(C.dump.p);
+    const char *old = static_cast<const char *>(C.seed[C.cur_seed].p);
+    size_t fill = 0;
+    const size_t kd = std::min<size_t>(D, R);
+    if (kd) HIPCHK(hipMemcpyAsync(dst, dump, kd * fbytes, hipMemcpyDeviceToDevice, E.side));
+    fill = kd;
+    const size_t ku = std::min<size_t>(U, R - fill);
+    if (ku) HIPCHK(hipMemcpyAsync(dst + fill * fbytes, old + size_t(ns) * fbytes, ku * fbytes, hipMemcpyDeviceToDevice, E.side));
+    fill += ku;
+    const size_t sd = D - kd, su = U - ku;
+    if (sd + su) {
+        if ((rc = grow_spill(C, C.spill_n + sd + su, fbytes))) return rc;
+        char *hs = reinterpret_cast<char *>(C.spill) + C.spill_n * fbytes;
+        if (sd) HIPCHK(hipMemcpyAsync(hs, dump + kd * fbytes, sd * fbytes, hipMemcpyDeviceToHost, E.side));
+        if (su)
+            HIPCHK(hipMemcpyAsync(hs + sd * fbytes, old + (size_t(ns) + ku) * fbytes, su * fbytes, hipMemcpyDeviceToHost, E.side));
+        C.spill_n += sd + su;
+        es.spilled += sd + su;
+        E.total_spilled += sd + su;
+    } else if (fill < R && C.spill_n) {
+        const size_t kr = std::min(R - fill, C.spill_n);
+        const char *hs = reinterpret_cast<const char *>(C.spill) + (C.spill_n - kr) * fbytes;
+        HIPCHK(hipMemcpyAsync(dst + fill * fbytes, hs, kr * fbytes, hipMemcpyHostToDevice, E.side));
+        C.spill_n -= kr;
+        fill += kr;
+        es.refilled += kr;
+    }
+    HIPCHK(hipStreamSynchronize(E.side));
+    C.cur_seed = 1 - C.cur_seed;
+    C.nseed = uint32_t(fill);
+    return KP_OK;
+}
+
+} // namespace
+
+int Engine::launch_epoch(const SearchOpts &o)
+{
+    // the class streams start after everything queued on the main stream
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev, stream));
+    int rc = KP_OK;
+    // Classes with the most pending work (sum of n^2 over roots not yet
+    // started, resumed frames counted at the class's largest n) launch first:
+    // each launch fills the device, and later ones start on the CUs that the
+    // earlier ones free as they drain.
+    std::vector<std::pair<double, int>> order;
+    for (int c = 0; c < kNumClass; c++) {
+        cls[c].launched = false;
+        if (!cls[c].has_work()) continue;
+        const ClassState &C = cls[c];
+        double w = double(C.nseed + C.spill_n) * double(C.maxn) * double(C.maxn);
+        for (size_t t = C.todo_done; t < C.htodo.size(); t++) {
+            const double n = double(C.hdesc[C.htodo[t]].n);
+            w += n * n;
+        }
+        order.emplace_back(-w, c);
+    }
+    std::stable_sort(order.begin(), order.end());
+    for (size_t oi = 0; oi < order.size() && !rc; oi++) {
+        const int c = order[oi].second;
+        if (hipStreamWaitEvent(cls[c].stream, ev, 0) != hipSuccess) rc = KP_EHIP;
+        else rc = launch_class_w(*this, c, o);
+        if (!rc && std::getenv("KP_SERIAL_CLASSES") && hipStreamSynchronize(cls[c].stream) != hipSuccess) rc = KP_EHIP;
+    }
+    (void)hipEventDestroy(ev);
+    return rc;
+}
+
+int Engine::finish_epoch(const SearchOpts &o, EpochStats &es)
+{
+    int first_err = KP_OK;
+    std::vector<std::pair<int, Sched>> got;
+    for (int c = 0; c < kNumClass; c++) {
+        ClassState &C = cls[c];
+        if (!C.launched) continue;
+        Sched hs{};
+        if (hipMemcpyAsync(&hs, C.sched.p, sizeof(Sched), hipMemcpyDeviceToHost, C.stream) != hipSuccess ||
+            hipStreamSynchronize(C.stream) != hipSuccess) {
+            set_error("search kernel (class W=%d) failed: %s", kClassW[c], hipGetErrorString(hipGetLastError()));
+            if (!first_err) first_err = KP_EHIP;
+            continue;
+        }
+        got.emplace_back(c, hs);
+    }
+    for (auto &cg : got) {
+        const int c = cg.first;
+        const Sched &hs = cg.second;
+        ClassState &C = cls[c];
+        C.launched = false;
+        if (std::getenv("KP_STATS"))
+            std::fprintf(stderr,
+                         "[kp] search W=%d todo=%u seeds=%u waves=%u (%d blocks/CU) ring=%u back=%u frames=%llu "
+                         "donated=%llu pops=%llu spills=%llu busy=%.1f%% wave-s=%.1f pause=%u dumped=%u\n",
+                         kClassW[c], C.launch_todo, C.nseed, C.nwaves, C.blocks_per_cu, C.ring, C.back,
+                         (unsigned long long)hs.frames, (unsigned long long)hs.donated, (unsigned long long)hs.pops,
+                         (unsigned long long)hs.spills,
+                         100.0 * double(hs.busy_ticks) / double(std::max<uint64_t>(1, hs.busy_ticks + hs.idle_ticks)),
+                         double(hs.busy_ticks + hs.idle_ticks) / 1e8, hs.pause, hs.dump_used);
+        if (hs.error) {
+            if (!first_err) {
+                set_error("search kernel (class W=%d) reported error %d", kClassW[c], hs.error);
+                first_err = hs.error;
+            }
+            continue;
+        }
+        es.frames += hs.frames;
+        es.donated += hs.donated;
+        const uint32_t started = std::min(hs.next_root, C.launch_todo);
+        const uint32_t ns = std::min(hs.next_seed, C.nseed);
+        C.todo_done += started;
+        if (!hs.pause && (started < C.launch_todo || ns < C.nseed || hs.dump_used)) {
+            if (!first_err) {
+                set_error("search kernel (class W=%d) ended with work left", kClassW[c]);
+                first_err = KP_EINVAL;
+            }
+            continue;
+        }
+        if (hs.pause) {
+            es.paused = true;
+            total_pauses++;
+        }
+        const uint32_t D = hs.pause ? hs.dump_used : 0;
+        int rc = carry_frames(*this, C, size_t(FrameWords(kClassW[c])) * 8, D, ns, o.seed_budget, es);
+        if (rc && !first_err) first_err = rc;
+    }
+    return first_err;
+}
+
+} // namespace kp
+
+// ----------------------------------------------------------------------
+// prepared graphs (kp_prepare / kp_run) and batches (kp_enumerate_batch)
+// ----------------------------------------------------------------------
 struct kp_prepared {
     kp_params p{};
     kp::Prep prep;
-    uint32_t pn = 0;
-    uint64_t pm = 0;
-    uint32_t maxdeg = 0, d1max = 0;
+    kp::DevBuf gbuf;
+    kp::GraphView gv;
+    kp::Engine E;
     double preprocess_ms = 0, upload_ms = 0;
-    hipStream_t stream = nullptr;
-    int device = 0;
-    int cus = 0;
-    // graph in HBM
-    DevBuf off, nbr, deg, dpos, dseq, orig;
-    // per-run buffers (reused across runs)
-    DevBuf blk, lft, info, ws_cnt, ws_b1c, ws_a1, desc, arena, wr_extra, list, list_used;
-    // one scheduler / queue / backing stack per size class (their kernels run concurrently)
-    DevBuf cq[kNumClass], cseq[kNumClass], csched[kNumClass], cstack[kNumClass], csubq[kNumClass];
-    hipStream_t cstream[kNumClass] = {};
-    uint32_t lcap = 0;
-    ~kp_prepared()
-    {
-        DevBuf *all[] = {&off, &nbr, &deg, &dpos, &dseq, &orig, &blk, &lft, &info, &ws_cnt, &ws_b1c, &ws_a1,
-                         &desc, &arena, &wr_extra, &list, &list_used};
-        for (DevBuf *b : all) b->release();
-        for (int c = 0; c < kNumClass; c++) {
-            cq[c].release();
-            cseq[c].release();
-            csched[c].release();
-            cstack[c].release();
-            csubq[c].release();
-            if (cstream[c]) (void)hipStreamDestroy(cstream[c]);
-        }
-        if (stream) (void)hipStreamDestroy(stream);
-    }
 };
 
 namespace kp {
@@ -1485,76 +2305,25 @@ int dev_info(char *buf, size_t len)
     return KP_OK;
 }
 
-uint64_t dev_bytes(const kp_prepared *h)
-{
-    if (!h) return 0;
-    const DevBuf *all[] = {&h->off, &h->nbr, &h->deg, &h->dpos, &h->dseq, &h->orig, &h->blk, &h->lft, &h->info,
-                           &h->ws_cnt, &h->ws_b1c, &h->ws_a1, &h->desc, &h->arena, &h->wr_extra, &h->list, &h->list_used};
-    uint64_t s = 0;
-    for (const DevBuf *b : all) s += b->bytes;
-    for (int c = 0; c < kNumClass; c++)
-        s += h->cq[c].bytes + h->cseq[c].bytes + h->csched[c].bytes + h->cstack[c].bytes + h->csubq[c].bytes;
-    return s;
-}
+uint64_t dev_bytes(const kp_prepared *h) { return h ? h->gbuf.bytes + h->E.device_bytes() : 0; }
 
 void dev_free(kp_prepared *h) { delete h; }
 
 int dev_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out)
 {
     *out = nullptr;
-    int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
-        set_error("no HIP device visible: the HIP backend is required");
-        return KP_ENODEV;
-    }
     auto *h = new (std::nothrow) kp_prepared();
     if (!h) return KP_ENOMEM;
     h->p = *p;
-    if (hipGetDevice(&h->device) != hipSuccess) { delete h; return KP_EHIP; }
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) { delete h; return KP_EHIP; }
-    h->cus = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return KP_EHIP; }
-    for (int c = 0; c < kNumClass; c++)
-        if (hipStreamCreateWithFlags(&h->cstream[c], hipStreamNonBlocking) != hipSuccess) { delete h; return KP_EHIP; }
-
-    const double t0 = now_ms();
-    int rc = preprocess(g, p->k, p->q, p->truss, &h->prep);
+    int rc = h->E.init();
     if (rc) { delete h; return rc; }
+    const double t0 = now_ms();
+    if ((rc = preprocess(g, p->k, p->q, p->truss, &h->prep))) { delete h; return rc; }
     h->preprocess_ms = now_ms() - t0;
-    const kp_graph &G = h->prep.g;
-    h->pn = G.n;
-    h->pm = G.m;
-    uint32_t maxdeg = 0, d1 = 0;
-    for (uint32_t v = 0; v < G.n; v++) {
-        maxdeg = std::max(maxdeg, G.deg[v]);
-        uint32_t later = 0;
-        for (uint64_t e = G.off[v]; e < G.off[v + 1]; e++) later += h->prep.dpos[G.nbr[e]] > h->prep.dpos[v];
-        d1 = std::max(d1, later);
-    }
-    h->maxdeg = maxdeg;
-    h->d1max = d1;
-
     const double t1 = now_ms();
-    const size_t pn = std::max<size_t>(G.n, 1);
-    if ((rc = h->off.ensure(sizeof(uint64_t) * (pn + 1))) || (rc = h->nbr.ensure(sizeof(uint32_t) * std::max<uint64_t>(G.m, 1))) ||
-        (rc = h->deg.ensure(4 * pn)) || (rc = h->dpos.ensure(4 * pn)) || (rc = h->dseq.ensure(4 * pn)) ||
-        (rc = h->orig.ensure(4 * pn))) {
-        delete h;
-        return rc;
-    }
-    auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
-        if (!bytes) return KP_OK;
-        return hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, h->stream) == hipSuccess ? KP_OK : KP_EHIP;
-    };
-    if ((rc = up(h->off, G.off, sizeof(uint64_t) * (size_t(G.n) + 1))) || (rc = up(h->nbr, G.nbr, 4 * G.m)) ||
-        (rc = up(h->deg, G.deg, 4 * size_t(G.n))) || (rc = up(h->dpos, h->prep.dpos.data(), 4 * size_t(G.n))) ||
-        (rc = up(h->dseq, h->prep.dseq.data(), 4 * size_t(G.n))) || (rc = up(h->orig, h->prep.orig.data(), 4 * size_t(G.n)))) {
-        set_error("upload to HBM failed");
-        delete h;
-        return rc;
-    }
-    if (hipStreamSynchronize(h->stream) != hipSuccess) { delete h; return KP_EHIP; }
+    std::vector<GraphView> views;
+    if ((rc = upload_graphs({&h->prep}, h->gbuf, views, h->E.stream))) { delete h; return rc; }
+    h->gv = views[0];
     h->upload_ms = now_ms() - t1;
     *out = h;
     return KP_OK;
@@ -1562,92 +2331,157 @@ int dev_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out)
 
 namespace {
 
-template <int WC>
-int launch_build(kp_prepared *h, const BuildArgs &ba, uint32_t nwg)
+// root positions [0, pn - q + 2) (src/device_funcs.cu:44-45), optionally windowed
+void root_window(uint32_t pn, const kp_params &P, int64_t &b, int64_t &e)
 {
-    hipLaunchKernelGGL((build_kernel<WC>), dim3(nwg), dim3(256), 0, h->stream, ba);
-    HIPCHK(hipGetLastError());
+    e = std::max<int64_t>(0, int64_t(pn) - P.q + 2);
+    b = 0;
+    if (P.root_end > 0 || P.root_begin > 0) {
+        b = std::min<int64_t>(P.root_begin, e);
+        if (P.root_end > 0) e = std::min<int64_t>(P.root_end, e);
+        e = std::max(e, b);
+    }
+}
+
+SearchOpts search_opts(const Engine &E, const kp_params &P, int device, int &rc)
+{
+    SearchOpts o;
+    rc = KP_OK;
+    o.want_list = (P.flags & KP_FLAG_LIST) != 0;
+    o.list_cap = o.want_list ? (64ull << 20) : 0;
+    if (P.time_limit_s > 0) {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) rc = KP_EHIP;
+        o.limit_ticks = uint64_t(P.time_limit_s * double(khz) * 1e3);
+    }
+    o.pause_frames = P.pause_every;
+    o.pausable = P.pause_every > 0;
+    const double t = (P.thres > 0 && P.thres <= 1) ? double(P.thres) : 1.0;
+    const uint64_t cap = P.queue_capacity ? P.queue_capacity : kDefaultQueueCap;
+    o.seed_budget = std::max<uint64_t>(1, uint64_t(t * double(cap)));
+    (void)E;
+    return o;
+}
+
+// Plex records are (len, job, ids...); each job's plexes become sorted id
+// lists in lexicographic order.
+int decode_lists(const std::vector<uint32_t> &listed, const std::vector<uint32_t> &job_slot, std::vector<kp_result *> &res)
+{
+    std::vector<std::vector<std::pair<uint64_t, uint32_t>>> recs(res.size());
+    std::vector<uint32_t> buf(listed);
+    for (uint64_t i = 0; i + 1 < buf.size();) {
+        const uint32_t len = buf[i], job = buf[i + 1];
+        if (i + 2 + len > buf.size()) break;
+        std::sort(buf.begin() + i + 2, buf.begin() + i + 2 + len);
+        if (job < job_slot.size() && job_slot[job] != ~0u) recs[job_slot[job]].emplace_back(i + 2, len);
+        i += 2 + len;
+    }
+    for (size_t j = 0; j < res.size(); j++) {
+        auto &rv = recs[j];
+        std::sort(rv.begin(), rv.end(), [&](const auto &x, const auto &y) {
+            return std::lexicographical_compare(buf.begin() + x.first, buf.begin() + x.first + x.second,
+                                                buf.begin() + y.first, buf.begin() + y.first + y.second);
+        });
+        kp_result *r = res[j];
+        r->nlisted = rv.size();
+        r->list_off = static_cast<uint64_t *>(std::malloc(8 * (rv.size() + 1)));
+        uint64_t tot = 0;
+        for (auto &x : rv) tot += x.second;
+        r->list_ids = static_cast<uint32_t *>(std::malloc(4 * std::max<uint64_t>(tot, 1)));
+        if (!r->list_off || !r->list_ids) return KP_ENOMEM;
+        uint64_t o = 0;
+        for (size_t i = 0; i < rv.size(); i++) {
+            r->list_off[i] = o;
+            std::memcpy(r->list_ids + o, buf.data() + rv[i].first, 4 * rv[i].second);
+            o += rv[i].second;
+        }
+        r->list_off[rv.size()] = o;
+    }
     return KP_OK;
 }
 
-struct ClassLaunch {
-    int W = 0;
-    uint32_t nwaves = 0, ring = 0, back = 0;
-    int blocks_per_cu = 0;
-};
-
-template <int W, int LW>
-int launch_search(kp_prepared *h, int c, SearchArgs sa, uint32_t maxn, ClassLaunch *cl)
+// Runs the jobs already in E's job table (ids 0..n-1) to completion: stage,
+// then search epochs until no class holds work.
+int run_pool(Engine &E, const std::vector<JobIn> &jobs, const kp_params &P0, std::vector<kp_result *> &res,
+             float &decompose_ms, float &search_ms)
 {
-    constexpr int FW = FrameLayout<W>::FW;
-    const uint32_t need = maxn + 3;   // DFS depth bound: every child shrinks C2 or C
-    const uint32_t ring = std::min<uint32_t>(need, std::max<uint32_t>(16, 8192 / (FW * 8)));
-    sa.ring_cap = ring;
-    sa.back_cap = need > ring ? need - ring + 1 : 0;
-    const size_t dyn = size_t(ring) * FW * 8 * 4;
-    auto kern = search_kernel<W, LW>;
-    if (dyn > 64 * 1024) HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn)));
-    int nb = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, dyn));
-    if (nb < 1) nb = 1;
-    const uint32_t nwg = uint32_t(nb) * uint32_t(h->cus);
-    sa.nwaves = nwg * 4;
-    sa.qcap = std::max<uint32_t>(256, sa.nwaves * 4 / NSUBQ);   // slots per shard
-    sa.donate_period = 8;
-    sa.donate_target = std::max<uint32_t>(16, sa.nwaves / 4);
-    const size_t nslots = size_t(sa.qcap) * NSUBQ;
     int rc;
-    if ((rc = h->cq[c].ensure(nslots * FW * 8)) || (rc = h->cseq[c].ensure(nslots * 8)) ||
-        (rc = h->csched[c].ensure(sizeof(Sched))) || (rc = h->csubq[c].ensure(sizeof(SubQ) * NSUBQ)))
-        return rc;
-    if (sa.back_cap && (rc = h->cstack[c].ensure(size_t(sa.nwaves) * sa.back_cap * FW * 8))) return rc;
-    sa.qframes = h->cq[c].as<uint64_t>();
-    sa.qseq = h->cseq[c].as<uint64_t>();
-    sa.sched = h->csched[c].as<Sched>();
-    sa.subq = h->csubq[c].as<SubQ>();
-    sa.gstack = sa.back_cap ? h->cstack[c].as<uint64_t>() : nullptr;
-    hipStream_t st = h->cstream[c];
-    HIPCHK(hipMemsetAsync(sa.sched, 0, sizeof(Sched), st));
-    HIPCHK(hipMemsetAsync(sa.subq, 0, sizeof(SubQ) * NSUBQ, st));
-    hipLaunchKernelGGL(iota_kernel, dim3(uint32_t((nslots + 255) / 256)), dim3(256), 0, st, sa.qseq, uint32_t(nslots),
-                       sa.qcap);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(kern, dim3(nwg), dim3(256), dyn, st, sa);
-    HIPCHK(hipGetLastError());
-    cl->W = W;
-    cl->nwaves = sa.nwaves;
-    cl->ring = sa.ring_cap;
-    cl->back = sa.back_cap;
-    cl->blocks_per_cu = nb;
+    E.reset_tables();
+    if ((rc = E.zero_acc(0, uint32_t(jobs.size())))) return rc;
+    hipEvent_t e0, e1, e2;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreate(&e2));
+    struct EvGuard {
+        hipEvent_t a, b, c;
+        ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); (void)hipEventDestroy(c); }
+    } eg{e0, e1, e2};
+    HIPCHK(hipEventRecord(e0, E.stream));
+    std::vector<JobStats> st;
+    // plain and batched runs: one arena segment per staging chunk
+    auto alloc = [&E](size_t, uint64_t words) -> uint64_t * {
+        auto *b = new (std::nothrow) DevBuf();
+        if (!b) return nullptr;
+        if (b->ensure(8 * words) || hipMemsetAsync(b->p, 0, 8 * words, E.stream) != hipSuccess) {
+            delete b;
+            return nullptr;
+        }
+        E.segs.push_back(b);
+        return b->as<uint64_t>();
+    };
+    if ((rc = E.stage(jobs, st, alloc))) return rc;
+    HIPCHK(hipEventRecord(e1, E.stream));
+    int orc = KP_OK;
+    SearchOpts o = search_opts(E, P0, E.device, orc);
+    if (orc) return orc;
+    if (o.want_list) {
+        if ((rc = E.list.ensure(o.list_cap * 4))) return rc;
+    }
+    if ((rc = E.list_used.ensure(8))) return rc;
+    HIPCHK(hipMemsetAsync(E.list_used.p, 0, 8, E.stream));
+    EpochStats es;
+    const uint64_t spilled0 = E.total_spilled, pauses0 = E.total_pauses;
+    while (E.has_work()) {
+        if ((rc = E.launch_epoch(o))) return rc;
+        if ((rc = E.finish_epoch(o, es))) return rc;
+    }
+    HIPCHK(hipEventRecord(e2, E.stream));
+    HIPCHK(hipEventSynchronize(e2));
+    HIPCHK(hipEventElapsedTime(&decompose_ms, e0, e1));
+    HIPCHK(hipEventElapsedTime(&search_ms, e1, e2));
+    std::vector<uint64_t> a5;
+    if ((rc = E.read_acc(0, uint32_t(jobs.size()), a5))) return rc;
+    for (size_t i = 0; i < jobs.size(); i++) {
+        kp_result *r = res[i];
+        r->count = a5[5 * i];
+        r->hash_a = a5[5 * i + 1];
+        r->hash_b = a5[5 * i + 2];
+        r->tasks = a5[5 * i + 3];
+        r->search_alg_bytes = a5[5 * i + 4];
+        r->alg_bytes = st[i].alg_bytes + r->search_alg_bytes;
+        r->adj_bytes = st[i].adj_bytes;
+        r->valid_roots = st[i].valid_roots;
+        r->max_n = st[i].max_n;
+        r->max_l = st[i].max_l;
+        r->pn = jobs[i].g.pn;
+        r->pm = jobs[i].g.pm;
+        r->spilled = E.total_spilled - spilled0;
+        r->pauses = E.total_pauses - pauses0;
+    }
+    if (o.want_list) {
+        uint64_t used = 0;
+        HIPCHK(hipMemcpy(&used, E.list_used.p, 8, hipMemcpyDeviceToHost));
+        if (used > o.list_cap) {
+            set_error("plex list exceeds %llu words", (unsigned long long)o.list_cap);
+            return KP_EOVERFLOW;
+        }
+        std::vector<uint32_t> listed(used);
+        if (used) HIPCHK(hipMemcpy(listed.data(), E.list.p, 4 * used, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> slot(jobs.size());
+        for (size_t i = 0; i < jobs.size(); i++) slot[i] = uint32_t(i);
+        if ((rc = decode_lists(listed, slot, res))) return rc;
+    }
     return KP_OK;
-}
-
-int run_search_class(kp_prepared *h, int c, SearchArgs sa, uint32_t maxn, ClassLaunch *cl)
-{
-    switch (kClassW[c]) {
-    case 1: return launch_search<1, 2>(h, c, sa, maxn, cl);
-    case 2: return launch_search<2, 2>(h, c, sa, maxn, cl);
-    case 3: return launch_search<3, 2>(h, c, sa, maxn, cl);
-    case 4: return launch_search<4, 1>(h, c, sa, maxn, cl);
-    case 6: return launch_search<6, 0>(h, c, sa, maxn, cl);
-    case 8: return launch_search<8, 0>(h, c, sa, maxn, cl);
-    case 16: return launch_search<16, 0>(h, c, sa, maxn, cl);
-    }
-    return KP_EINVAL;
-}
-
-int run_build_class(kp_prepared *h, int c, const BuildArgs &ba, uint32_t nwg)
-{
-    switch (kClassW[c]) {
-    case 1: return launch_build<1>(h, ba, nwg);
-    case 2: return launch_build<2>(h, ba, nwg);
-    case 3: return launch_build<3>(h, ba, nwg);
-    case 4: return launch_build<4>(h, ba, nwg);
-    case 6: return launch_build<6>(h, ba, nwg);
-    case 8: return launch_build<8>(h, ba, nwg);
-    case 16: return launch_build<16>(h, ba, nwg);
-    }
-    return KP_EINVAL;
 }
 
 } // namespace
@@ -1655,275 +2489,71 @@ int run_build_class(kp_prepared *h, int c, const BuildArgs &ba, uint32_t nwg)
 int dev_run(kp_prepared *h, kp_result *res)
 {
     std::memset(res, 0, sizeof(*res));
-    const kp_params &P = h->p;
-    HIPCHK(hipSetDevice(h->device));
-    const kp_graph &G = h->prep.g;
-    res->pn = h->pn;
-    res->pm = h->pm;
+    HIPCHK(hipSetDevice(h->E.device));
+    int rc;
+    if ((rc = h->E.reserve_jobs(1))) return rc;
+    h->E.set_job(0, h->gv, h->p.k, h->p.q);
+    if ((rc = h->E.sync_jobs())) return rc;
+    JobIn j{0, h->gv, h->p.k, h->p.q, 0, 0};
+    root_window(h->gv.pn, h->p, j.pos_begin, j.pos_end);
+    std::vector<kp_result *> rv{res};
+    float dms = 0, sms = 0;
+    if ((rc = run_pool(h->E, {j}, h->p, rv, dms, sms))) return rc;
     res->preprocess_ms = h->preprocess_ms;
     res->upload_ms = h->upload_ms;
-    // roots at degeneracy positions [0, pn - q + 2) (src/device_funcs.cu:44-45), optionally windowed
-    int64_t pos_end = std::max<int64_t>(0, int64_t(h->pn) - P.q + 2);
-    int64_t pos_begin = 0;
-    if (P.root_end > 0 || P.root_begin > 0) {
-        pos_begin = std::min<int64_t>(P.root_begin, pos_end);
-        if (P.root_end > 0) pos_end = std::min<int64_t>(P.root_end, pos_end);
-        pos_end = std::max(pos_end, pos_begin);
-    }
-    const int64_t nroots = pos_end - pos_begin;
-    hipEvent_t e0, e1, e2;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventCreate(&e2));
-    struct EvGuard { hipEvent_t a, b, c; ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); (void)hipEventDestroy(c); } } eg{e0, e1, e2};
-    HIPCHK(hipEventRecord(e0, h->stream));
-    int rc;
-    std::vector<RootInfo> info(static_cast<size_t>(nroots));
-    std::vector<RootDesc> descs;
-    std::vector<int> dclass;
-    std::vector<uint32_t> class_begin(kNumClass + 1, 0), class_maxn(kNumClass, 0);
-    std::vector<uint64_t> wr_extra;
-    if (nroots > 0) {
-        // ---- decomposition ----
-        h->lcap = std::min<uint32_t>(std::max<uint32_t>(h->pn, 1), std::max<uint32_t>(2048, 2 * h->maxdeg + 1024));
-        const uint32_t d1cap = std::max<uint32_t>(h->d1max, 1);
-        const uint32_t w1cap = (d1cap + 63) / 64;
-        uint32_t nwaves = uint32_t(std::min<int64_t>(nroots, int64_t(h->cus) * 16));
-        // keep the per-wave counter arrays under ~8 GiB
-        const uint64_t cnt_bytes = uint64_t(std::max<uint32_t>(h->pn, 1)) * 4;
-        nwaves = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(nwaves, (8ull << 30) / cnt_bytes)));
-        const uint32_t nwg = (nwaves + WPB - 1) / WPB;
-        if ((rc = h->blk.ensure(size_t(nroots) * NCAP * 4)) || (rc = h->lft.ensure(size_t(nroots) * h->lcap * 4)) ||
-            (rc = h->info.ensure(size_t(nroots) * sizeof(RootInfo))) || (rc = h->ws_cnt.ensure(size_t(nwaves) * cnt_bytes)) ||
-            (rc = h->ws_b1c.ensure(size_t(nwaves) * d1cap * 4)) || (rc = h->ws_a1.ensure(size_t(nwaves) * d1cap * w1cap * 8)))
-            return rc;
-        HIPCHK(hipMemsetAsync(h->ws_cnt.p, 0, size_t(nwaves) * cnt_bytes, h->stream));
-        DecompArgs da{};
-        da.off = h->off.as<uint64_t>();
-        da.nbr = h->nbr.as<uint32_t>();
-        da.deg = h->deg.as<uint32_t>();
-        da.dpos = h->dpos.as<int32_t>();
-        da.dseq = h->dseq.as<int32_t>();
-        da.pn = std::max<uint32_t>(h->pn, 1);
-        da.k = P.k;
-        da.q = P.q;
-        da.pos_begin = pos_begin;
-        da.nroots = nroots;
-        da.blk = h->blk.as<uint32_t>();
-        da.lft = h->lft.as<uint32_t>();
-        da.lcap = h->lcap;
-        da.info = h->info.as<RootInfo>();
-        da.ws_cnt = h->ws_cnt.as<uint32_t>();
-        da.ws_b1c = h->ws_b1c.as<uint32_t>();
-        da.ws_a1 = h->ws_a1.as<uint64_t>();
-        da.d1cap = d1cap;
-        da.w1cap = w1cap;
-        da.nwaves = nwaves;
-        hipLaunchKernelGGL(decompose_kernel, dim3(nwg), dim3(256), 0, h->stream, da);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(info.data(), da.info, sizeof(RootInfo) * size_t(nroots), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
+    res->decompose_ms = dms;
+    res->search_ms = sms;
+    res->wall_ms = res->preprocess_ms + dms + sms;
+    return KP_OK;
+}
 
-        // ---- classes and arena layout ----
-        std::vector<std::vector<uint32_t>> by_class(kNumClass);
-        for (int64_t pos = 0; pos < nroots; pos++) {
-            const RootInfo &ri = info[size_t(pos)];
-            if (ri.status < 0) {
-                if (ri.status == -1) {
-                    kp::set_error("root at degeneracy position %lld: block exceeds %u vertices", (long long)(pos + pos_begin), NCAP);
-                    return KP_ERANGE;
-                }
-                kp::set_error("root at degeneracy position %lld: left set exceeds %u vertices", (long long)(pos + pos_begin), h->lcap);
-                return KP_EOVERFLOW;
-            }
-            if (ri.status != 1) continue;
-            res->valid_roots++;
-            res->max_n = std::max(res->max_n, ri.n);
-            res->max_l = std::max(res->max_l, ri.l);
-            by_class[class_of(ri.n)].push_back(uint32_t(pos));
-        }
-        uint64_t arena_words = 0;
-        for (int c = 0; c < kNumClass; c++) {
-            auto &v = by_class[c];
-            // largest blocks first (longest-processing-time order)
-            std::stable_sort(v.begin(), v.end(), [&](uint32_t x, uint32_t y) { return info[x].n > info[y].n; });
-            class_begin[c] = uint32_t(descs.size());
-            const uint32_t W = uint32_t(kClassW[c]);
-            for (uint32_t pos : v) {
-                const RootInfo &ri = info[pos];
-                RootDesc d{arena_words, ri.n, ri.h, ri.l, pos};
-                descs.push_back(d);
-                dclass.push_back(c);
-                class_maxn[c] = std::max(class_maxn[c], ri.n);
-                arena_words += 3ull * ri.n * W + uint64_t(ri.l) * W + (ri.n + 1) / 2;
-            }
-        }
-        class_begin[kNumClass] = uint32_t(descs.size());
-        const uint32_t nd = uint32_t(descs.size());
-        if (nd) {
-            if ((rc = h->desc.ensure(sizeof(RootDesc) * nd)) || (rc = h->arena.ensure(8 * arena_words)) ||
-                (rc = h->wr_extra.ensure(8 * nd)))
-                return rc;
-            HIPCHK(hipMemcpyAsync(h->desc.p, descs.data(), sizeof(RootDesc) * nd, hipMemcpyHostToDevice, h->stream));
-            HIPCHK(hipMemsetAsync(h->arena.p, 0, 8 * arena_words, h->stream));
-            for (int c = 0; c < kNumClass; c++) {
-                const uint32_t b = class_begin[c], e = class_begin[c + 1];
-                if (b == e) continue;
-                BuildArgs ba{};
-                ba.off = da.off;
-                ba.nbr = da.nbr;
-                ba.deg = da.deg;
-                ba.orig = h->orig.as<uint32_t>();
-                ba.blk = da.blk;
-                ba.lft = da.lft;
-                ba.lcap = h->lcap;
-                ba.desc = h->desc.as<RootDesc>() + b;
-                ba.ndesc = e - b;
-                ba.arena = h->arena.as<uint64_t>();
-                ba.ws_cnt = da.ws_cnt;
-                ba.pn = da.pn;
-                ba.nwaves = std::min<uint32_t>(nwaves, ((e - b) + WPB - 1) / WPB * WPB);
-                ba.k = P.k;
-                ba.q = P.q;
-                ba.wr_extra = h->wr_extra.as<uint64_t>() + b;
-                if ((rc = run_build_class(h, c, ba, (ba.nwaves + WPB - 1) / WPB))) return rc;
-            }
-            wr_extra.resize(nd);
-            HIPCHK(hipMemcpyAsync(wr_extra.data(), h->wr_extra.p, 8 * nd, hipMemcpyDeviceToHost, h->stream));
-        }
+// Many graphs in one pool: host preprocessing in parallel over the graphs,
+// one packed upload, one staging pass over every root, one set of search
+// launches.  Each result's decompose/search times are the batch's.
+int dev_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r)
+{
+    for (size_t i = 0; i < n; i++) std::memset(&r[i], 0, sizeof(kp_result));
+    if (!n) return KP_OK;
+    Engine E;
+    int rc = E.init();
+    if (rc) return rc;
+    const double t0 = now_ms();
+    std::vector<Prep> preps(n);
+    std::vector<int> prc(n, KP_OK);
+    std::vector<double> pms(n, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t i = 0; i < n; i++) {
+        const double a = now_ms();
+        prc[i] = preprocess(g[i], p[i].k, p[i].q, p[i].truss, &preps[i]);
+        pms[i] = now_ms() - a;
     }
-    HIPCHK(hipEventRecord(e1, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    for (size_t i = 0; i < descs.size(); i++) {
-        const RootInfo &ri = info[descs[i].pos];
-        const uint64_t wr = ri.wr + wr_extra[i];
-        res->adj_bytes += 4 * wr;
-        res->alg_bytes += 4 * wr + (uint64_t(ri.n) * ri.n + 7) / 8 + uint64_t(ri.n) * ri.n;
+    for (size_t i = 0; i < n; i++)
+        if (prc[i]) return prc[i];
+    const double t1 = now_ms();
+    DevBuf gbuf;
+    std::vector<GraphView> views;
+    std::vector<const Prep *> pp(n);
+    for (size_t i = 0; i < n; i++) pp[i] = &preps[i];
+    if ((rc = upload_graphs(pp, gbuf, views, E.stream))) return rc;
+    const double t2 = now_ms();
+    if ((rc = E.reserve_jobs(n))) return rc;
+    std::vector<JobIn> jobs(n);
+    for (size_t i = 0; i < n; i++) {
+        E.set_job(uint32_t(i), views[i], p[i].k, p[i].q);
+        jobs[i] = JobIn{uint32_t(i), views[i], p[i].k, p[i].q, 0, 0};
+        root_window(views[i].pn, p[i], jobs[i].pos_begin, jobs[i].pos_end);
     }
-
-    // ---- persistent search: one launch per size class, all classes concurrently ----
-    const bool want_list = (P.flags & KP_FLAG_LIST) != 0;
-    const uint64_t list_cap = want_list ? (64ull << 20) : 0;
-    if (want_list && (rc = h->list.ensure(list_cap * 4))) return rc;
-    if ((rc = h->list_used.ensure(8))) return rc;
-    HIPCHK(hipMemsetAsync(h->list_used.p, 0, 8, h->stream));
-    uint64_t limit_ticks = 0;
-    if (P.time_limit_s > 0) {
-        int khz = 0;
-        HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device));
-        limit_ticks = uint64_t(P.time_limit_s * double(khz) * 1e3);
-    }
-    // every class stream starts after the build; the main stream waits for all of them
-    hipEvent_t ebuild;
-    HIPCHK(hipEventCreateWithFlags(&ebuild, hipEventDisableTiming));
-    HIPCHK(hipEventRecord(ebuild, h->stream));
-    std::vector<int> order;
-    for (int c = 0; c < kNumClass; c++)
-        if (class_begin[c + 1] > class_begin[c]) order.push_back(c);
-    // largest classes (by sum of n^2 over their roots) first
-    std::vector<double> cwork(kNumClass, 0.0);
-    for (size_t i = 0; i < descs.size(); i++) cwork[dclass[i]] += double(descs[i].n) * descs[i].n;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cwork[x] > cwork[y]; });
-    ClassLaunch launches[kNumClass];
-    std::vector<hipEvent_t> cdone;
-    const bool serial_classes = std::getenv("KP_SERIAL_CLASSES") != nullptr;
-    int launch_rc = KP_OK;
-    for (int c : order) {
-        const uint32_t b = class_begin[c], e = class_begin[c + 1];
-        HIPCHK(hipStreamWaitEvent(h->cstream[c], ebuild, 0));
-        SearchArgs sa{};
-        sa.desc = h->desc.as<RootDesc>() + b;
-        sa.ndesc = e - b;
-        sa.arena = h->arena.as<uint64_t>();
-        sa.k = P.k;
-        sa.q = P.q;
-        sa.list = want_list ? h->list.as<uint32_t>() : nullptr;
-        sa.list_cap = list_cap;
-        sa.list_used = h->list_used.as<uint64_t>();
-        sa.limit_ticks = limit_ticks;
-        if ((launch_rc = run_search_class(h, c, sa, class_maxn[c], &launches[c]))) break;
-        hipEvent_t ev;
-        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        HIPCHK(hipEventRecord(ev, h->cstream[c]));
-        HIPCHK(hipStreamWaitEvent(h->stream, ev, 0));
-        cdone.push_back(ev);
-        if (serial_classes) HIPCHK(hipStreamSynchronize(h->cstream[c]));
-    }
-    HIPCHK(hipStreamSynchronize(h->stream));
-    for (int c : order) HIPCHK(hipStreamSynchronize(h->cstream[c]));
-    for (hipEvent_t ev : cdone) (void)hipEventDestroy(ev);
-    (void)hipEventDestroy(ebuild);
-    if (launch_rc) return launch_rc;
-    for (int c : order) {
-        Sched hs{};
-        HIPCHK(hipMemcpy(&hs, h->csched[c].p, sizeof(Sched), hipMemcpyDeviceToHost));
-        if (std::getenv("KP_STATS"))
-            std::fprintf(stderr,
-                         "[kp] search W=%d roots=%u waves=%u (%d blocks/CU) ring=%u back=%u frames=%llu tasks=%llu "
-                         "donated=%llu pops=%llu spills=%llu busy=%.1f%% wave-s=%.1f\n",
-                         kClassW[c], class_begin[c + 1] - class_begin[c], launches[c].nwaves,
-                         launches[c].blocks_per_cu, launches[c].ring, launches[c].back, (unsigned long long)hs.frames,
-                         (unsigned long long)hs.tasks, (unsigned long long)hs.donated, (unsigned long long)hs.pops,
-                         (unsigned long long)hs.spills,
-                         100.0 * double(hs.busy_ticks) / double(std::max<uint64_t>(1, hs.busy_ticks + hs.idle_ticks)),
-                         double(hs.busy_ticks + hs.idle_ticks) / 1e8);
-        if (hs.error) {
-            kp::set_error("search kernel (class W=%d) reported error %d", kClassW[c], hs.error);
-            return hs.error;
-        }
-        res->count += hs.count;
-        res->hash_a += hs.ha;
-        res->hash_b += hs.hb;
-        res->tasks += hs.tasks;
-        res->alg_bytes += hs.alg;
-        res->search_alg_bytes += hs.alg;
-    }
-    std::vector<uint32_t> listed;
-    if (want_list) {
-        uint64_t used = 0;
-        HIPCHK(hipMemcpy(&used, h->list_used.p, 8, hipMemcpyDeviceToHost));
-        if (used > list_cap) {
-            kp::set_error("plex list exceeds %llu words", (unsigned long long)list_cap);
-            return KP_EOVERFLOW;
-        }
-        listed.resize(used);
-        if (used) HIPCHK(hipMemcpy(listed.data(), h->list.p, 4 * used, hipMemcpyDeviceToHost));
-    }
-    HIPCHK(hipEventRecord(e2, h->stream));
-    HIPCHK(hipEventSynchronize(e2));
-    float ms1 = 0, ms2 = 0;
-    HIPCHK(hipEventElapsedTime(&ms1, e0, e1));
-    HIPCHK(hipEventElapsedTime(&ms2, e1, e2));
-    res->decompose_ms = ms1;
-    res->search_ms = ms2;
-    res->wall_ms = res->preprocess_ms + ms1 + ms2;
-    (void)G;
-    if (want_list) {
-        // records are (len, ids...); sort ids within a plex, then plexes lexicographically
-        std::vector<std::pair<uint64_t, uint32_t>> recs;
-        for (uint64_t i = 0; i < listed.size();) {
-            const uint32_t len = listed[i];
-            std::sort(listed.begin() + i + 1, listed.begin() + i + 1 + len);
-            recs.emplace_back(i + 1, len);
-            i += 1 + len;
-        }
-        std::sort(recs.begin(), recs.end(), [&](const auto &x, const auto &y) {
-            return std::lexicographical_compare(listed.begin() + x.first, listed.begin() + x.first + x.second,
-                                                listed.begin() + y.first, listed.begin() + y.first + y.second);
-        });
-        res->nlisted = recs.size();
-        res->list_off = static_cast<uint64_t *>(std::malloc(8 * (recs.size() + 1)));
-        uint64_t tot = 0;
-        for (auto &r : recs) tot += r.second;
-        res->list_ids = static_cast<uint32_t *>(std::malloc(4 * std::max<uint64_t>(tot, 1)));
-        if (!res->list_off || !res->list_ids) return KP_ENOMEM;
-        uint64_t o = 0;
-        for (size_t i = 0; i < recs.size(); i++) {
-            res->list_off[i] = o;
-            std::memcpy(res->list_ids + o, listed.data() + recs[i].first, 4 * recs[i].second);
-            o += recs[i].second;
-        }
-        res->list_off[recs.size()] = o;
+    if ((rc = E.sync_jobs())) return rc;
+    std::vector<kp_result *> rv(n);
+    for (size_t i = 0; i < n; i++) rv[i] = &r[i];
+    float dms = 0, sms = 0;
+    if ((rc = run_pool(E, jobs, p[0], rv, dms, sms))) return rc;
+    for (size_t i = 0; i < n; i++) {
+        r[i].preprocess_ms = pms[i];
+        r[i].upload_ms = t2 - t1;
+        r[i].decompose_ms = dms;
+        r[i].search_ms = sms;
+        r[i].wall_ms = (t1 - t0) + dms + sms;
     }
     return KP_OK;
 }

@@ -1,0 +1,145 @@
+// kp_engine.h -- host side of the device pool shared by the plain, batched
+// and served entry points (compiled with hipcc only).
+//
+// A job is (preprocessed graph resident in HBM, k, q, window of degeneracy
+// positions).  The engine decomposes the roots of any set of jobs, builds
+// their subgraphs, files the valid roots into per-size-class root tables and
+// runs the persistent search kernels over all of them in epochs.  An epoch
+// ends when every class has drained, or when a pause was requested (a frame
+// budget, or the host pause word): then the pending frames of each class are
+// carried into the next epoch, at most `seed_budget` of them on the device and
+// the rest in pinned host memory.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kp_internal.h"
+
+#include <functional>
+#include <vector>
+
+namespace kpd {
+struct RootDesc;
+struct JobDev;
+struct Sched;
+} // namespace kpd
+
+namespace kp {
+
+constexpr int kNumClass = 7;
+extern const int kClassW[kNumClass];   // bitset words per size class
+int class_of(uint32_t n);              // -1 when n > 64 * 16
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);            // grows (contents are not kept)
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(p); }
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+};
+
+// A preprocessed graph in HBM (local ids 0..pn-1) plus host-side sizes.
+struct GraphView {
+    const uint64_t *off = nullptr;
+    const uint32_t *nbr = nullptr, *deg = nullptr, *orig = nullptr;
+    const int32_t *dpos = nullptr, *dseq = nullptr;
+    uint32_t pn = 0;
+    uint64_t pm = 0;
+    uint32_t maxdeg = 0, d1max = 0;   // largest degree / largest later-neighbour count
+};
+
+// Packs preprocessed graphs into one device allocation (one host->HBM copy).
+size_t packed_graph_bytes(const Prep &p);
+int upload_graphs(const std::vector<const Prep *> &preps, DevBuf &buf, std::vector<GraphView> &views,
+                  hipStream_t st);
+
+struct JobIn {
+    uint32_t id;               // index in the engine's job table
+    GraphView g;
+    int k, q;
+    int64_t pos_begin, pos_end;   // root positions, already clipped to [0, pn - q + 2)
+};
+
+struct JobStats {
+    uint64_t valid_roots = 0, adj_bytes = 0, alg_bytes = 0;
+    uint32_t max_n = 0, max_l = 0;
+    uint64_t arena_bytes = 0;
+};
+
+struct SearchOpts {
+    bool want_list = false;
+    uint64_t list_cap = 0;        // words
+    uint64_t limit_ticks = 0;
+    uint64_t pause_frames = 0;    // frame budget per launch (0 = none)
+    bool host_pause = false;      // waves poll the engine's pinned pause word
+    bool pausable = false;        // allocate dump arrays (needed by either pause source)
+    uint64_t seed_budget = 1;     // resumed frames kept on the device per class
+};
+
+struct ClassState {
+    DevBuf q, qseq, sched, stack, subq, desc, todo, seed[2], dump;
+    hipStream_t stream = nullptr;
+    int blocks_per_cu = 0;
+    std::vector<kpd::RootDesc> hdesc;   // root table (frames name roots by index)
+    std::vector<uint32_t> htodo;        // start list
+    size_t todo_done = 0;               // entries of htodo already started
+    size_t desc_uploaded = 0;           // entries of hdesc copied to the device
+    uint32_t maxn = 0;
+    int cur_seed = 0;
+    uint32_t nseed = 0;                 // resumed frames in seed[cur_seed]
+    uint64_t *spill = nullptr;          // pinned host frames (a stack)
+    size_t spill_cap = 0, spill_n = 0;  // in frames
+    // last launch
+    bool launched = false;
+    uint32_t nwaves = 0, ring = 0, back = 0, dump_cap = 0, launch_todo = 0;
+    hipEvent_t done_ev = nullptr;
+    bool has_work() const { return todo_done < htodo.size() || nseed > 0 || spill_n > 0; }
+    ~ClassState();
+};
+
+struct EpochStats {
+    uint64_t frames = 0, donated = 0, spilled = 0, refilled = 0;
+    bool paused = false;
+};
+
+struct Engine {
+    int device = 0, cus = 0;
+    hipStream_t stream = nullptr, side = nullptr;
+    DevBuf jobs, acc;                  // JobDev table, JobAcc per job
+    std::vector<kpd::JobDev> hjobs;
+    size_t jobs_uploaded = 0;
+    DevBuf blk, lft, info, slot_job, slot_pos, ws_cnt, ws_b1c, ws_a1, bdesc, wr_extra;
+    size_t ws_cnt_zeroed = 0;          // bytes of ws_cnt known to be zero
+    std::vector<DevBuf *> segs;        // arena segments owned by the engine (plain / batch)
+    DevBuf list, list_used;
+    ClassState cls[kNumClass];
+    uint32_t *hpause = nullptr, *dpause = nullptr;   // pinned host pause word
+    uint64_t total_spilled = 0, total_pauses = 0;
+
+    int init();
+    ~Engine();
+    uint64_t device_bytes() const;
+    // job table: make room for `n` jobs, set entry i (uploaded by sync_jobs)
+    int reserve_jobs(size_t n);
+    void set_job(uint32_t id, const GraphView &g, int k, int q);
+    int sync_jobs();
+    int zero_acc(uint32_t first, uint32_t count);
+    // Decomposes and builds every root of `jobs`; appends the valid roots of
+    // each class (largest first) to its root table and start list.  `alloc`
+    // returns zeroed device memory for `words` 64-bit words of job index i.
+    int stage(const std::vector<JobIn> &jobs, std::vector<JobStats> &st,
+              const std::function<uint64_t *(size_t i, uint64_t words)> &alloc);
+    // One epoch: launch every class with work; finish waits and carries pauses.
+    int launch_epoch(const SearchOpts &o);
+    int finish_epoch(const SearchOpts &o, EpochStats &es);
+    bool has_work() const;
+    void reset_tables();               // forget every root table, start list and pending frame
+    int read_acc(uint32_t first, uint32_t count, std::vector<uint64_t> &out5);   // 5 words per job
+};
+
+int dev_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r);
+
+} // namespace kp

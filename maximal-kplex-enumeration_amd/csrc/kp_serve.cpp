@@ -10,16 +10,6 @@
 
 using namespace kp;
 
-extern "C" int kp_enumerate_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r)
-{
-    if ((!g || !p || !r) && n) return KP_EINVAL;
-    for (size_t i = 0; i < n; i++) {
-        int rc = kp_enumerate(g[i], &p[i], &r[i]);
-        if (rc) return rc;
-    }
-    return KP_OK;
-}
-
 struct kp_server {
     kp_server_config cfg{};
     std::vector<kp_graph> graphs;

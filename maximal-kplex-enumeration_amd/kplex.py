@@ -39,7 +39,7 @@ class KpGraph(ctypes.Structure):
 class KpParams(ctypes.Structure):
     _fields_ = [("k", c_int), ("q", c_int), ("thres", c_float), ("truss", c_int), ("flags", c_uint32),
                 ("time_limit_s", c_double), ("queue_capacity", c_uint64), ("root_begin", ctypes.c_int64),
-                ("root_end", ctypes.c_int64)]
+                ("root_end", ctypes.c_int64), ("pause_every", c_uint64)]
 
 
 class KpResult(ctypes.Structure):
@@ -233,18 +233,23 @@ def _result_dict(r, listed):
     return d
 
 
-def make_params(k, q, thres=0.25, truss=0, flags=0, time_limit_s=0.0, queue_capacity=0, roots=None):
+def make_params(k, q, thres=0.25, truss=0, flags=0, time_limit_s=0.0, queue_capacity=0, roots=None,
+                pause_every=0):
     rb, re_ = roots if roots else (0, 0)
     return KpParams(k=k, q=q, thres=thres, truss=truss, flags=flags, time_limit_s=time_limit_s,
-                    queue_capacity=queue_capacity, root_begin=rb, root_end=re_)
+                    queue_capacity=queue_capacity, root_begin=rb, root_end=re_, pause_every=pause_every)
 
 
-def enumerate_plexes(graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0, validate=False, roots=None):
+def enumerate_plexes(graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0, validate=False, roots=None,
+                     queue_capacity=0, pause_every=0):
     """kp_enumerate: returns a dict with count, hash, tasks, timings (and plexes when listing).
-    roots=(b, e) restricts the enumeration to roots at degeneracy positions [b, e)."""
+    roots=(b, e) restricts the enumeration to roots at degeneracy positions [b, e).
+    pause_every > 0 pauses each search launch after about that many frames and
+    resumes it; thres * queue_capacity frames stay on the device, the rest spill
+    to pinned host memory."""
     lib = load()
     flags = (KP_FLAG_LIST if listing else 0) | (KP_FLAG_VALIDATE if validate else 0)
-    p = make_params(k, q, thres, truss, flags, time_limit_s, roots=roots)
+    p = make_params(k, q, thres, truss, flags, time_limit_s, queue_capacity, roots=roots, pause_every=pause_every)
     r = KpResult()
     _check(lib.kp_enumerate(ctypes.byref(graph.g), ctypes.byref(p), ctypes.byref(r)), "kp_enumerate")
     try:
@@ -253,14 +258,42 @@ def enumerate_plexes(graph, k, q, thres=0.25, truss=0, listing=False, time_limit
         lib.kp_result_free(ctypes.byref(r))
 
 
+def enumerate_batch(graphs, params, listing=False):
+    """kp_enumerate_batch: graphs[i] with params[i] = (k, q) or (k, q, truss) or a
+    dict of make_params arguments; all roots share one pool of launches.
+    Returns one result dict per graph."""
+    lib = load()
+    n = len(graphs)
+    ps = (KpParams * max(n, 1))()
+    for i, p in enumerate(params):
+        if isinstance(p, dict):
+            kw = dict(p)
+        else:
+            kw = dict(k=p[0], q=p[1], truss=p[2] if len(p) > 2 else 0)
+        kw.setdefault("flags", KP_FLAG_LIST if listing else 0)
+        ps[i] = make_params(**kw)
+    gp = (POINTER(KpGraph) * max(n, 1))()
+    for i, g in enumerate(graphs):
+        gp[i] = ctypes.pointer(g.g)
+    rs = (KpResult * max(n, 1))()
+    try:
+        _check(lib.kp_enumerate_batch(gp, ps, n, rs), "kp_enumerate_batch")
+        return [_result_dict(rs[i], listing) for i in range(n)]
+    finally:
+        for i in range(n):
+            lib.kp_result_free(ctypes.byref(rs[i]))
+
+
 class Prepared:
     """kp_prepare / kp_run: the graph preprocessed and resident in HBM."""
 
-    def __init__(self, graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0, roots=None):
+    def __init__(self, graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0, roots=None,
+                 queue_capacity=0, pause_every=0):
         self.lib = load()
         self.h = c_void_p()
         self.listing = listing
-        p = make_params(k, q, thres, truss, KP_FLAG_LIST if listing else 0, time_limit_s, roots=roots)
+        p = make_params(k, q, thres, truss, KP_FLAG_LIST if listing else 0, time_limit_s, queue_capacity,
+                        roots=roots, pause_every=pause_every)
         _check(self.lib.kp_prepare(ctypes.byref(graph.g), ctypes.byref(p), ctypes.byref(self.h)), "kp_prepare")
 
     def run(self):

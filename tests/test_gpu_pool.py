@@ -1,0 +1,138 @@
+"""GPU tests of the shared pool: many graphs in one set of launches
+(kp_enumerate_batch) and pause / resume with pinned-host spill.  Every
+result must equal the plain one-graph run and the oracle."""
+import sys
+
+import pytest
+
+import kplex
+import kpo
+from kp_testutil import csr, disjoint_cliques, multipartite, random_graph
+
+pytestmark = pytest.mark.gpu
+
+LIMIT = 90.0
+KEYS = ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l")
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def same(a, b, keys=KEYS):
+    for x in keys:
+        assert a[x] == b[x], (x, a[x], b[x])
+
+
+def oracle(graph, k, q, truss=0, listing=False):
+    deg, nbr = graph.csr()
+    return kpo.enumerate_plexes(kpo.OGraph.from_csr(deg, nbr), k, q, truss, listing=listing)
+
+
+def mixed_graphs():
+    """Uneven graphs and parameters: closed forms, random, planted groups."""
+    out = []
+    out.append((kplex.Graph.from_csr(*csr(multipartite(4, 3))), 2, 8, 0))
+    out.append((kplex.Graph.from_csr(*csr(disjoint_cliques([5, 6, 70, 130]))), 2, 4, 0))
+    for s in range(4):
+        out.append((kplex.Graph.from_csr(*csr(random_graph(120 + 20 * s, 0.3, 500 + s))), 2 + s % 2, 5 + s, 0))
+    out.append((kplex.Graph.er_planted(1200, 0.01, 6, 30, 3, 9), 3, 14, 0))
+    out.append((kplex.Graph.er_planted(1500, 0.004, 2, 140, 1, 180), 2, 60, 0))
+    out.append((kplex.Graph.er_planted(1500, 0.012, 8, 28, 3, 21), 3, 16, 1))
+    out.append((kplex.Graph.from_csr(*csr(multipartite(2, 2))), 2, 8, 0))   # no valid root
+    return out
+
+
+def test_batch_equals_plain_and_oracle():
+    cases = mixed_graphs()
+    say(f"batch of {len(cases)}")
+    rs = kplex.enumerate_batch([g for g, *_ in cases], [(k, q, tr) for _, k, q, tr in cases])
+    for (g, k, q, tr), r in zip(cases, rs):
+        say(f"  plain n={g.n} k={k} q={q}")
+        p = kplex.enumerate_plexes(g, k, q, truss=tr, time_limit_s=LIMIT)
+        same(r, p)
+        o = oracle(g, k, q, tr)
+        same(r, o)
+
+
+def test_batch_lists_split_per_graph():
+    cases = mixed_graphs()[:7]
+    say("batch lists")
+    rs = kplex.enumerate_batch([g for g, *_ in cases], [(k, q, tr) for _, k, q, tr in cases], listing=True)
+    for (g, k, q, tr), r in zip(cases, rs):
+        o = oracle(g, k, q, tr, listing=True)
+        assert [tuple(p) for p in r["plexes"]] == o["plexes"]
+        assert r["nlisted"] == r["count"]
+
+
+def test_batch_of_one_and_same_graph_twice():
+    g = kplex.Graph.er_planted(1200, 0.01, 6, 30, 3, 9)
+    say("batch duplicates")
+    rs = kplex.enumerate_batch([g, g, g], [(3, 14), (3, 14), (2, 10)])
+    same(rs[0], rs[1])
+    same(rs[2], kplex.enumerate_plexes(g, 2, 10, time_limit_s=LIMIT))
+
+
+def test_batch_many_chung_lu_graphs():
+    # small members of the many-graph workload family: uneven sizes and k in {2,3,4}
+    specs = kplex.batch_graph_specs(count=24, nmax=1500, seed=11)
+    graphs = [kplex.graph_from_spec(s) for s in specs]
+    say(f"batch chung-lu x{len(graphs)}")
+    rs = kplex.enumerate_batch(graphs, [(s["k"], s["q"]) for s in specs])
+    tot = 0
+    for s, g, r in zip(specs, graphs, rs):
+        p = kplex.enumerate_plexes(g, s["k"], s["q"], time_limit_s=LIMIT)
+        same(r, p)
+        tot += r["count"]
+    assert tot > 0
+
+
+def mid_graph():
+    """12 planted 40-vertex groups with 8 removed edges per vertex: k=3, q=18
+    gives 6,524 maximal 3-plexes over 2.5 million Branch tasks."""
+    return kplex.Graph.er_planted(3000, 0.004, 12, 40, 8, 7), 3, 18
+
+
+def wide_graph():
+    """Planted 70-vertex groups (the two-word class): k=2, q=20, 8 million tasks."""
+    return kplex.Graph.er_planted(2000, 0.004, 6, 70, 14, 5), 2, 20
+
+
+@pytest.mark.parametrize("pause_every,cap,thres", [(20000, 64, 0.25), (200000, 1 << 20, 0.25), (50000, 8, 1.0)])
+def test_pause_resume_identical(pause_every, cap, thres):
+    g, k, q = mid_graph()
+    say(f"pause every {pause_every} cap {cap} thres {thres}")
+    base = kplex.enumerate_plexes(g, k, q, time_limit_s=LIMIT)
+    r = kplex.enumerate_plexes(g, k, q, time_limit_s=LIMIT, thres=thres, queue_capacity=cap,
+                               pause_every=pause_every)
+    same(r, base)
+    assert r["pauses"] > 0
+    if cap * thres < 1000:
+        assert r["spilled"] > 0
+    else:
+        assert r["spilled"] == 0
+
+
+def test_pause_resume_lists():
+    g, k, q = mid_graph()
+    say("pause lists")
+    r = kplex.enumerate_plexes(g, k, q, listing=True, time_limit_s=LIMIT, queue_capacity=16, pause_every=30000)
+    o = oracle(g, k, q, listing=True)
+    same(r, o)
+    assert [tuple(p) for p in r["plexes"]] == o["plexes"]
+    assert r["pauses"] > 0 and r["spilled"] > 0
+
+
+def test_batch_with_pauses():
+    cases = mixed_graphs()
+    for g, k, q in (mid_graph(), wide_graph()):
+        cases.append((g, k, q, 0))
+    say("batch + pauses")
+    ps = [dict(k=k, q=q, truss=tr, pause_every=100000, queue_capacity=32) for _, k, q, tr in cases]
+    rs = kplex.enumerate_batch([g for g, *_ in cases], ps)
+    base = kplex.enumerate_batch([g for g, *_ in cases], [(k, q, tr) for _, k, q, tr in cases])
+    for r, b in zip(rs, base):
+        same(r, b)
+    assert rs[0]["pauses"] > 0 and rs[0]["spilled"] > 0
+    same(rs[-2], oracle(*mid_graph()))
