@@ -148,18 +148,41 @@ void kp_prepared_free(kp_prepared *h);
 int  kp_enumerate_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r);
 
 /* ---- served: continuous batching of (graph, k, q) queries ----------- */
+/* No reference counterpart.  Registered graphs stay in host memory; each
+ * (graph, q-k, truss) is preprocessed once and its CSR + degeneracy order stay
+ * resident in HBM.  Queries run in epochs of the persistent search launches:
+ * while queries wait, the running epoch is paused after slice_ms (every wave
+ * writes its pending frames out), the waiting queries are admitted (root
+ * decomposition + subgraph build) as job slots and arena memory allow, and
+ * the next epoch resumes the paused frames together with the new roots.  A
+ * query's result is the same alone or in any mix. */
 typedef struct kp_server kp_server;
 typedef struct kp_server_config {
-    size_t   max_inflight;      /* queries on the device at once (0 = default) */
-    uint64_t mem_budget_bytes;  /* admission budget for projected stack/queue bytes (0 = default) */
+    size_t   max_inflight;      /* queries admitted at once (0 = 64) */
+    uint64_t mem_budget_bytes;  /* admission budget: subgraph arena bytes of admitted queries
+                                   (0 = 4 GiB); the search stacks and queues are a fixed
+                                   per-size-class cost, not per query */
+    double   slice_ms;          /* minimum epoch length before a pause for waiting queries
+                                   (0 = 2 ms) */
 } kp_server_config;
 int  kp_server_create(const kp_server_config *cfg, kp_server **out);
-/* Registers a graph; it stays resident in HBM until the server is destroyed. */
+/* Registers a graph (copied). */
 int  kp_server_add_graph(kp_server *s, const kp_graph *g, uint32_t *graph_id);
-/* Queues a query; returns its ticket.  Admission happens in kp_server_poll. */
+/* Preprocesses (graph_ids[i], p[i].k, p[i].q, p[i].truss) for i < n, in
+ * parallel on the host, and keeps each result resident in HBM, so that later
+ * queries on them start with decomposition. */
+int  kp_server_preload(kp_server *s, const uint32_t *graph_ids, const kp_params *p, size_t n);
+/* Queues a query (k, q, truss of *p; lists and root windows are not served);
+ * returns its ticket.  Admission happens in kp_server_poll. */
 int  kp_server_submit(kp_server *s, uint32_t graph_id, const kp_params *p, uint64_t *ticket);
-/* Drives the server; returns completed queries (up to cap) in tickets/results. */
+/* One scheduling step, never blocking on the device: finishes an epoch that
+ * has ended, admits waiting queries, starts the next epoch, requests a pause
+ * when queries wait.  Returns up to cap completed queries; in each result
+ * wall_ms is submit -> completion, search_ms admission -> completion and
+ * decompose_ms the admission (decompose + build) time of its group. */
 int  kp_server_poll(kp_server *s, uint64_t *tickets, kp_result *results, size_t cap, size_t *ndone);
+/* Number of queries submitted and not yet returned by kp_server_poll. */
+size_t kp_server_outstanding(const kp_server *s);
 int  kp_server_destroy(kp_server *s);
 
 /* ---- misc ------------------------------------------------------------ */

@@ -46,54 +46,6 @@ constexpr uint32_t CLAIM = 0x40000000u;
 constexpr uint32_t CMASK = 0x3FFFFFFFu;
 constexpr uint32_t SPIN_LIMIT = 1u << 24;
 
-struct RootInfo {          // decomposition result per (job, root position)
-    int32_t status;        // 1 valid, 0 not, <0 error
-    uint32_t n, h, l;
-    uint64_t wr;           // CSR entries read: d(r) + sum over later neighbours, then + sum over B and L
-};
-
-// One job's graph in HBM (local vertex ids 0..pn-1) and its parameters.
-struct JobDev {
-    const uint64_t *off;
-    const uint32_t *nbr, *deg, *orig;
-    const int32_t *dpos, *dseq;
-    uint32_t pn;
-    int32_t k, q;
-    uint32_t pad;
-};
-
-// Per-job result accumulators (sums mod 2^64), one 64-byte line each.
-struct JobAcc {
-    unsigned long long count, ha, hb, tasks, alg, pad[3];
-};
-static_assert(sizeof(JobAcc) == 64, "JobAcc is one line");
-
-struct RootDesc {          // one valid root; frames name roots by their index in the class table
-    uint64_t addr;         // device address of the root's subgraph
-    uint32_t n, h, l, slot;
-    uint32_t job;
-    uint16_t k, q;
-};
-static_assert(sizeof(RootDesc) == 32, "RootDesc layout");
-
-struct Sched {             // persistent-kernel scheduler state (one per launch)
-    uint32_t next_root;    // cursor over the start list
-    uint32_t next_seed;    // cursor over the resumed frames
-    uint32_t active;
-    uint32_t q_count;      // published frames over all sub-queues
-    uint32_t hungry;       // waves currently looking for work
-    uint32_t pause;        // set once a pause is requested; waves dump and exit
-    uint32_t exited;
-    uint32_t dump_used;    // frames written to the dump array
-    int32_t error;
-    uint32_t pad0;
-    uint64_t epoch_frames; // frames processed, in steps of the check interval
-    uint64_t list_unused;
-    uint64_t donated;
-    uint64_t frames, spills, pops;
-    uint64_t busy_ticks, idle_ticks;   // wall-clock ticks summed over waves
-};
-
 // One shard of the frame queue: a bounded MPMC ring with sequence-numbered
 // slots.  Shards sit on separate 128-byte lines so that waves polling and
 // claiming different shards do not contend on one address.
@@ -2212,6 +2164,48 @@ int Engine::launch_epoch(const SearchOpts &o)
     }
     (void)hipEventDestroy(ev);
     return rc;
+}
+
+int Engine::enable_host_pause()
+{
+    if (hpause) return KP_OK;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, 256, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        set_error("pinned pause word allocation failed");
+        return KP_ENOMEM;
+    }
+    hpause = static_cast<uint32_t *>(p);
+    *hpause = 0;
+    void *d = nullptr;
+    HIPCHK(hipHostGetDevicePointer(&d, p, 0));
+    dpause = static_cast<uint32_t *>(d);
+    return KP_OK;
+}
+
+bool Engine::epoch_done()
+{
+    for (const auto &c : cls)
+        if (c.launched && hipEventQuery(c.done_ev) == hipErrorNotReady) return false;
+    return true;
+}
+
+int Engine::count_pending(std::vector<uint64_t> &per_job)
+{
+    std::vector<uint64_t> meta;
+    for (int c = 0; c < kNumClass; c++) {
+        ClassState &C = cls[c];
+        const size_t fb = size_t(FrameWords(kClassW[c])) * 8;
+        meta.assign(C.nseed, 0);
+        if (C.nseed)
+            HIPCHK(hipMemcpy2DAsync(meta.data(), 8, C.seed[C.cur_seed].p, fb, 8, C.nseed, hipMemcpyDeviceToHost, side));
+        HIPCHK(hipStreamSynchronize(side));
+        for (size_t i = 0; i < C.spill_n; i++) meta.push_back(C.spill[i * (fb / 8)]);
+        for (uint64_t m : meta) {
+            const uint32_t root = uint32_t(m);
+            if (root < C.hdesc.size() && C.hdesc[root].job < per_job.size()) per_job[C.hdesc[root].job]++;
+        }
+    }
+    return KP_OK;
 }
 
 int Engine::finish_epoch(const SearchOpts &o, EpochStats &es)

@@ -13,15 +13,10 @@
 #include <hip/hip_runtime.h>
 
 #include "kp_internal.h"
+#include "kp_layout.h"
 
 #include <functional>
 #include <vector>
-
-namespace kpd {
-struct RootDesc;
-struct JobDev;
-struct Sched;
-} // namespace kpd
 
 namespace kp {
 
@@ -138,6 +133,11 @@ struct Engine {
     bool has_work() const;
     void reset_tables();               // forget every root table, start list and pending frame
     int read_acc(uint32_t first, uint32_t count, std::vector<uint64_t> &out5);   // 5 words per job
+    // pending frames (device seeds and host spill) per job index; per_job must have room
+    int count_pending(std::vector<uint64_t> &per_job);
+    // allocates the pinned, device-mapped pause word that SearchOpts::host_pause polls
+    int enable_host_pause();
+    bool epoch_done();                 // every launched class has finished (does not block)
 };
 
 int dev_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r);

@@ -52,7 +52,7 @@ class KpResult(ctypes.Structure):
 
 
 class KpServerConfig(ctypes.Structure):
-    _fields_ = [("max_inflight", c_size_t), ("mem_budget_bytes", c_uint64)]
+    _fields_ = [("max_inflight", c_size_t), ("mem_budget_bytes", c_uint64), ("slice_ms", c_double)]
 
 
 # every entry point include/kplex.h declares: name -> (restype, argtypes)
@@ -73,6 +73,8 @@ SIGNATURES = {
     "kp_server_add_graph": (c_int, [c_void_p, POINTER(KpGraph), POINTER(c_uint32)]),
     "kp_server_submit": (c_int, [c_void_p, c_uint32, POINTER(KpParams), POINTER(c_uint64)]),
     "kp_server_poll": (c_int, [c_void_p, POINTER(c_uint64), POINTER(KpResult), c_size_t, POINTER(c_size_t)]),
+    "kp_server_outstanding": (c_size_t, [c_void_p]),
+    "kp_server_preload": (c_int, [c_void_p, POINTER(c_uint32), POINTER(KpParams), c_size_t]),
     "kp_server_destroy": (c_int, [c_void_p]),
     "kp_strerror": (c_char_p, [c_int]),
     "kp_last_error": (c_char_p, []),
@@ -310,6 +312,59 @@ class Prepared:
     def close(self):
         if self.h:
             self.lib.kp_prepared_free(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """kp_server_*: continuous batching of (graph, k, q) queries."""
+
+    def __init__(self, max_inflight=64, mem_budget_bytes=0, slice_ms=0.0):
+        self.lib = load()
+        self.h = c_void_p()
+        cfg = KpServerConfig(max_inflight=max_inflight, mem_budget_bytes=mem_budget_bytes, slice_ms=slice_ms)
+        _check(self.lib.kp_server_create(ctypes.byref(cfg), ctypes.byref(self.h)), "kp_server_create")
+
+    def add_graph(self, graph):
+        gid = c_uint32()
+        _check(self.lib.kp_server_add_graph(self.h, ctypes.byref(graph.g), ctypes.byref(gid)), "kp_server_add_graph")
+        return gid.value
+
+    def preload(self, items):
+        """items: [(graph_id, k, q)] or [(graph_id, k, q, truss)]; preprocessed in parallel, kept in HBM."""
+        n = len(items)
+        ids = (c_uint32 * max(n, 1))()
+        ps = (KpParams * max(n, 1))()
+        for i, it in enumerate(items):
+            ids[i] = it[0]
+            ps[i] = make_params(it[1], it[2], truss=it[3] if len(it) > 3 else 0)
+        _check(self.lib.kp_server_preload(self.h, ids, ps, n), "kp_server_preload")
+
+    def submit(self, graph_id, k, q, truss=0):
+        t = c_uint64()
+        p = make_params(k, q, truss=truss)
+        _check(self.lib.kp_server_submit(self.h, graph_id, ctypes.byref(p), ctypes.byref(t)), "kp_server_submit")
+        return t.value
+
+    def poll(self, cap=256):
+        """One scheduling step; returns [(ticket, result dict)] of completed queries."""
+        ts = (c_uint64 * cap)()
+        rs = (KpResult * cap)()
+        nd = c_size_t()
+        _check(self.lib.kp_server_poll(self.h, ts, rs, cap, ctypes.byref(nd)), "kp_server_poll")
+        return [(ts[i], _result_dict(rs[i], False)) for i in range(nd.value)]
+
+    def outstanding(self):
+        return self.lib.kp_server_outstanding(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.kp_server_destroy(self.h)
             self.h = c_void_p()
 
     def __del__(self):

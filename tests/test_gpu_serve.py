@@ -1,0 +1,136 @@
+"""GPU tests of the served mode (kp_server_*): queries admitted into running
+epochs must give the same results as the plain one-graph runs, alone or in
+any mix, under slot and memory limits."""
+import sys
+import time
+
+import pytest
+
+import kplex
+
+pytestmark = pytest.mark.gpu
+
+LIMIT = 90.0
+KEYS = ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l")
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def same(a, b, keys=KEYS):
+    for x in keys:
+        assert a[x] == b[x], (x, a[x], b[x])
+
+
+def graphs_and_queries():
+    graphs = [kplex.Graph.er_planted(3000, 0.004, 12, 40, 8, 7),
+              kplex.Graph.er_planted(2000, 0.004, 6, 70, 14, 5),
+              kplex.Graph.er_planted(1200, 0.01, 6, 30, 3, 9)]
+    specs = kplex.batch_graph_specs(count=6, nmax=3000, seed=4, kset=(3, 4, 5), qfun=lambda k: 5 * k + 10)
+    graphs += [kplex.graph_from_spec(s) for s in specs]
+    queries = [(0, 3, 18), (1, 2, 20), (2, 3, 14), (0, 2, 12), (2, 2, 10), (1, 3, 22)]
+    queries += [(3 + i, s["k"], s["q"]) for i, s in enumerate(specs)]
+    queries += [(0, 3, 18), (3, specs[0]["k"], specs[0]["q"])]   # repeats share the resident graph
+    return graphs, queries
+
+
+def drain(srv, want, timeout_s=120.0):
+    got = {}
+    t0 = time.time()
+    while len(got) < want:
+        for t, r in srv.poll():
+            got[t] = r
+        assert time.time() - t0 < timeout_s, f"server did not finish: {len(got)}/{want}"
+        time.sleep(0.0002)
+    return got
+
+
+def plain(graphs, q):
+    g, k, qq = graphs[q[0]], q[1], q[2]
+    return kplex.enumerate_plexes(g, k, qq, time_limit_s=LIMIT)
+
+
+def test_server_staggered_arrivals_equal_plain():
+    graphs, queries = graphs_and_queries()
+    srv = kplex.Server(max_inflight=6, slice_ms=0.5)
+    ids = [srv.add_graph(g) for g in graphs]
+    tickets = {}
+    say(f"server: {len(queries)} queries, staggered")
+    got = {}
+    for i, (gi, k, q) in enumerate(queries):
+        tickets[srv.submit(ids[gi], k, q)] = i
+        # let the running epoch make progress so that later queries arrive mid-epoch
+        for _ in range(3):
+            for t, r in srv.poll():
+                got[t] = r
+            time.sleep(0.001)
+    t0 = time.time()
+    while len(got) < len(queries):
+        for t, r in srv.poll():
+            got[t] = r
+        assert time.time() - t0 < 120, f"server did not finish: {len(got)}/{len(queries)}"
+        time.sleep(0.0002)
+    srv.close()
+    # queries arrived while epochs ran: some epochs were paused to admit them
+    assert max(r["pauses"] for r in got.values()) > 0
+    for t, i in tickets.items():
+        say(f"  plain {queries[i]}")
+        same(got[t], plain(graphs, queries[i]))
+
+
+def test_server_alone_and_in_mix():
+    graphs, queries = graphs_and_queries()
+    target = queries[11]   # the last Chung-Lu member
+    say(f"alone {target}")
+    srv = kplex.Server()
+    gid = srv.add_graph(graphs[target[0]])
+    t = srv.submit(gid, target[1], target[2])
+    alone = drain(srv, 1)[t]
+    srv.close()
+    say("in a mix")
+    srv = kplex.Server(max_inflight=4, slice_ms=0.2)
+    ids = [srv.add_graph(g) for g in graphs]
+    mix = [srv.submit(ids[gi], k, q) for gi, k, q in queries[:5]]
+    tt = srv.submit(ids[target[0]], target[1], target[2])
+    mix += [srv.submit(ids[gi], k, q) for gi, k, q in queries[5:9]]
+    got = drain(srv, len(mix) + 1)
+    srv.close()
+    same(got[tt], alone)
+    assert got[tt]["count"] > 0
+
+
+def test_server_memory_budget_admits_in_turn():
+    graphs, queries = graphs_and_queries()
+    # a budget that holds one or two subgraph arenas at a time
+    srv = kplex.Server(max_inflight=8, mem_budget_bytes=4 << 20, slice_ms=0.5)
+    ids = [srv.add_graph(g) for g in graphs]
+    say("small arena budget")
+    sub = queries[:6]
+    ts = [srv.submit(ids[gi], k, q) for gi, k, q in sub]
+    got = drain(srv, len(ts))
+    srv.close()
+    for t, qy in zip(ts, sub):
+        same(got[t], plain(graphs, qy))
+
+
+def test_server_query_larger_than_budget_is_refused():
+    graphs, queries = graphs_and_queries()
+    srv = kplex.Server(max_inflight=2, mem_budget_bytes=4096)
+    gid = srv.add_graph(graphs[1])
+    srv.submit(gid, 2, 20)
+    with pytest.raises(kplex.KplexError) as e:
+        for _ in range(1000):
+            srv.poll()
+            time.sleep(0.001)
+    assert e.value.code == kplex.KP_ENOMEM
+    srv.close()
+
+
+def test_server_rejects_lists_and_bad_params():
+    srv = kplex.Server()
+    gid = srv.add_graph(kplex.Graph.er_planted(300, 0.05, 1, 10, 0, 1))
+    with pytest.raises(kplex.KplexError):
+        srv.submit(gid, 3, 5)
+    srv.close()

@@ -18,7 +18,7 @@ from kp_testutil import csr, random_graph
 def header_functions():
     with open(os.path.join(ROOT, "include", "kplex.h")) as f:
         txt = f.read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char|uint64_t)\s*\*?\s*(kp_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char|uint64_t|size_t)\s*\*?\s*(kp_\w+)\s*\(", txt, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
