@@ -116,7 +116,7 @@ def batched_vs_plain(ngraphs, time_limit):
     kp_enumerate_batch call.  Results must be identical graph by graph."""
     import kplex
     specs = kplex.batch_graph_specs()[:ngraphs]
-    graphs = [kplex.graph_from_spec(s) for s in specs]
+    graphs = gen_graphs(specs)
     params = [(s["k"], s["q"]) for s in specs]
     log(f"[bench] batched: {len(graphs)} graphs, n {min(s['n'] for s in specs)}..{max(s['n'] for s in specs)}")
     kplex.enumerate_plexes(graphs[0], *params[0], time_limit_s=time_limit)   # warm the library
@@ -140,6 +140,85 @@ def batched_vs_plain(ngraphs, time_limit):
             "identical": True}
 
 
+def gen_graphs(specs):
+    """Generates the graphs of `specs` on host threads (the generator call
+    releases the GIL)."""
+    import concurrent.futures
+    import kplex
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(kplex.graph_from_spec, specs))
+
+
+def served_vs_plain(duration_s, rate, time_limit, seed=7):
+    """The served workload: 256 resident graphs built like the many-graph
+    workload but with n up to 200,000; queries (graph uniform, k in {3,4,5},
+    q = 5k+10) arrive as a Poisson stream of `rate` per second for
+    `duration_s` seconds.  The server admits them into running epochs
+    (continuous batching); latency = completion - arrival.  Then the same
+    queries run one after another through the plain entry point
+    (kp_enumerate per query).  Results must be identical query by query."""
+    import numpy as np
+    import kplex
+    specs = kplex.batch_graph_specs(count=256, nmax=200000, seed=4, kset=(3, 4, 5), qfun=lambda k: 5 * k + 10)
+    t0 = time.perf_counter()
+    graphs = gen_graphs(specs)
+    gen_s = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    arrivals, queries = [], []
+    t = rng.exponential(1.0 / rate)
+    while t < duration_s:
+        k = int(rng.choice([3, 4, 5]))
+        arrivals.append(t)
+        queries.append((int(rng.integers(len(graphs))), k, 5 * k + 10))
+        t += rng.exponential(1.0 / rate)
+    log(f"[bench] served: {len(graphs)} graphs generated in {gen_s:.1f} s; {len(queries)} queries at {rate}/s "
+        f"over {duration_s} s")
+    srv = kplex.Server(max_inflight=128, slice_ms=2.0)
+    ids = [srv.add_graph(g) for g in graphs]
+    t0 = time.perf_counter()
+    srv.preload([(ids[i], k, 5 * k + 10) for i in range(len(graphs)) for k in (3, 4, 5)])
+    preload_s = time.perf_counter() - t0
+    log(f"[bench] served: 768 (graph, k, q) preprocessed and resident in {preload_s:.1f} s")
+    done = {}
+    tick = {}
+    nxt = 0
+    start = time.perf_counter()
+    last_log = start
+    while len(done) < len(queries):
+        now = time.perf_counter() - start
+        while nxt < len(queries) and arrivals[nxt] <= now:
+            gi, k, q = queries[nxt]
+            tick[srv.submit(ids[gi], k, q)] = nxt
+            nxt += 1
+        got = srv.poll()
+        tnow = time.perf_counter() - start
+        for tk, r in got:
+            done[tick[tk]] = (tnow, r)
+        if time.perf_counter() - last_log > 10:
+            last_log = time.perf_counter()
+            log(f"[bench] served: {len(done)}/{len(queries)} done at {tnow:.1f} s")
+        if not got:
+            time.sleep(0.0001)
+    makespan = max(t for t, _ in done.values())
+    srv.close()
+    lat = sorted(done[i][0] - arrivals[i] for i in range(len(queries)))
+    pct = lambda p: lat[min(len(lat) - 1, int(p * len(lat)))]
+    count = sum(r["count"] for _, r in done.values())
+    log(f"[bench] served: makespan {makespan:.2f} s, p50 {pct(0.5) * 1e3:.0f} ms, p99 {pct(0.99) * 1e3:.0f} ms")
+    t0 = time.perf_counter()
+    for i, (gi, k, q) in enumerate(queries):
+        r = kplex.enumerate_plexes(graphs[gi], k, q, time_limit_s=time_limit)
+        for x in ("count", "hash", "tasks", "valid_roots"):
+            assert r[x] == done[i][1][x], ("served result differs", i, x, r[x], done[i][1][x])
+    plain_s = time.perf_counter() - t0
+    log(f"[bench] served: plain one-by-one {plain_s:.2f} s")
+    return {"graphs": len(graphs), "queries": len(queries), "offered_per_s": rate, "arrival_window_s": duration_s,
+            "plexes": count, "makespan_s": makespan, "served_queries_per_s": len(queries) / makespan,
+            "served_plexes_per_s": count / makespan, "p50_ms": pct(0.5) * 1e3, "p99_ms": pct(0.99) * 1e3,
+            "plain_s": plain_s, "plain_queries_per_s": len(queries) / plain_s, "served_vs_plain": plain_s / makespan,
+            "preload_s": preload_s, "identical": True}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +230,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=48,
                     help="graphs of the many-graph workload for the batched-vs-plain line (0 = skip)")
+    ap.add_argument("--served-seconds", type=float, default=1.5,
+                    help="arrival window of the served stream at --served-rate queries/s (0 = skip)")
+    ap.add_argument("--served-rate", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -221,6 +303,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(graph, k, q, truss, args.cpu_target_s, window)
     if args.batch_graphs > 0:
         out["batched"] = batched_vs_plain(args.batch_graphs, args.time_limit)
+    if args.served_seconds > 0:
+        out["served"] = served_vs_plain(args.served_seconds, args.served_rate, args.time_limit)
     print(json.dumps(out))
 
 
