@@ -233,6 +233,7 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=1.5,
                     help="arrival window of the served stream at --served-rate queries/s (0 = skip)")
     ap.add_argument("--served-rate", type=float, default=200.0)
+    ap.add_argument("--no-autotune", action="store_true", help="time with the built-in launch settings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -247,6 +248,15 @@ def main():
     t0 = time.perf_counter()
     prep = kplex.Prepared(graph, k, q, thres, truss, time_limit_s=args.time_limit, roots=window)
     prep_s = time.perf_counter() - t0
+    tune, tune_s, untuned_ms = None, 0.0, None
+    if not args.no_autotune:
+        r = prep.run()
+        r = prep.run()
+        untuned_ms = r["search_ms"]
+        t0 = time.perf_counter()
+        tune = prep.autotune()
+        tune_s = time.perf_counter() - t0
+        log(f"[bench] autotune {tune_s:.1f} s: {tune} (default settings: search {untuned_ms:.1f} ms)")
     for i in range(args.warmup):
         r = prep.run()
         log(f"[bench] warmup {i}: count={r['count']} search_ms={r['search_ms']:.1f} decompose_ms={r['decompose_ms']:.1f}")
@@ -296,8 +306,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.workload),
                      "kernel": "search_kernel", "alg_bytes_per_launch": r0["search_alg_bytes"]},
         "cpu_baseline": None,
+        "autotune": {"settings_by_class_words": tune, "tune_s": tune_s, "default_settings_search_ms": untuned_ms},
     }
     prep.close()
+    kplex.set_tune(None)   # the batched and served lines use the built-in settings
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline (target {args.cpu_target_s} s)")
         out["cpu_baseline"] = cpu_baseline(graph, k, q, truss, args.cpu_target_s, window)

@@ -139,6 +139,24 @@ int  kp_run(kp_prepared *h, kp_result *r);
 uint64_t kp_prepared_bytes(const kp_prepared *h);
 void kp_prepared_free(kp_prepared *h);
 
+/* ---- launch settings per size class (autotuner) --------------------- */
+/* No reference counterpart (the reference has fixed WARPS and queue sizes,
+ * inc/common.h).  Index c is the size class of blocks with at most 64 * W_c
+ * vertices, W = {1, 2, 3, 4, 6, 8, 16}; a 0 entry keeps the built-in default. */
+typedef struct kp_tune {
+    int32_t  blocks_per_cu[7];  /* resident 256-thread blocks per CU (<= occupancy limit) */
+    uint32_t donate_period[7];  /* frames between donation checks (default 8) */
+    uint32_t ring_frames[7];    /* frames per wave in the LDS stack ring (default 8 KiB worth) */
+    uint32_t queue_slots[7];    /* frame-queue slots per shard (default max(256, waves / 8)) */
+    double   class_ms[7];       /* kp_autotune: best search time of the class alone */
+} kp_tune;
+/* Times each size class of the prepared workload alone under candidate
+ * settings (coordinate descent) and makes the fastest the process-wide
+ * settings, which every later run, batch and server launch uses. */
+int  kp_autotune(kp_prepared *h, kp_tune *out);
+/* Sets the process-wide settings (NULL = defaults). */
+void kp_set_tune(const kp_tune *t);
+
 /* ---- batched: many graphs in one persistent launch ------------------ */
 /* No reference counterpart (the reference handles one graph per process).
  * p[i] holds graph i's k, q, truss and root window; flags, time limit and

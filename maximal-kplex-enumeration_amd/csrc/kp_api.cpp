@@ -83,6 +83,14 @@ extern "C" int kp_run(kp_prepared *h, kp_result *r)
 
 extern "C" uint64_t kp_prepared_bytes(const kp_prepared *h) { return dev_bytes(h); }
 
+extern "C" int kp_autotune(kp_prepared *h, kp_tune *out)
+{
+    if (!h || !out) return KP_EINVAL;
+    return dev_autotune(h, out);
+}
+
+extern "C" void kp_set_tune(const kp_tune *t) { dev_set_tune(t); }
+
 extern "C" void kp_prepared_free(kp_prepared *h) { dev_free(h); }
 
 extern "C" int kp_enumerate(const kp_graph *g, const kp_params *p, kp_result *r)

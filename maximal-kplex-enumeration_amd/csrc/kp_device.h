@@ -9,6 +9,8 @@ namespace kp {
 int  dev_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out);
 int  dev_run(kp_prepared *h, kp_result *r);
 int  dev_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r);
+int  dev_autotune(kp_prepared *h, kp_tune *out);
+void dev_set_tune(const kp_tune *t);
 void dev_free(kp_prepared *h);
 uint64_t dev_bytes(const kp_prepared *h);
 int  dev_info(char *buf, size_t len);

@@ -1536,6 +1536,7 @@ using namespace kpd;
 namespace kp {
 
 const int kClassW[kNumClass] = {1, 2, 3, 4, 6, 8, 16};
+ClassTune g_tune[kNumClass];
 
 int class_of(uint32_t n)
 {
@@ -1974,22 +1975,26 @@ int launch_class(Engine &E, int c, const SearchOpts &o)
 {
     ClassState &C = E.cls[c];
     constexpr int FW = FrameLayout<W>::FW;
+    const ClassTune &T = g_tune[c];
     const uint32_t need = C.maxn + 3;   // DFS depth bound: every child shrinks C2 or C
-    const uint32_t ring = std::min<uint32_t>(need, std::max<uint32_t>(16, 8192 / (FW * 8)));
+    // LDS ring: 8 KiB of frames per wave unless tuned; at most 64 KiB per block
+    const uint32_t ring_max = std::max<uint32_t>(4, (64 * 1024) / (FW * 8 * 4));
+    uint32_t ring = T.ring_frames ? std::max<uint32_t>(4, T.ring_frames) : std::max<uint32_t>(16, 8192 / (FW * 8));
+    ring = std::min(std::min(ring, ring_max), need);
     const uint32_t back = need > ring ? need - ring + 1 : 0;
     const size_t dyn = size_t(ring) * FW * 8 * 4;
     auto kern = search_kernel<W, LW>;
-    if (dyn > 64 * 1024) HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn)));
     int nb = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, dyn));
     if (nb < 1) nb = 1;
+    if (T.blocks_per_cu > 0) nb = std::min(nb, T.blocks_per_cu);
     const uint32_t nwg = uint32_t(nb) * uint32_t(E.cus);
     SearchArgs sa{};
     sa.nwaves = nwg * 4;
-    sa.qcap = std::max<uint32_t>(256, sa.nwaves * 4 / NSUBQ);   // slots per shard
+    sa.qcap = T.queue_slots ? std::max<uint32_t>(16, T.queue_slots) : std::max<uint32_t>(256, sa.nwaves * 4 / NSUBQ);
     sa.ring_cap = ring;
     sa.back_cap = back;
-    sa.donate_period = 8;
+    sa.donate_period = T.donate_period ? T.donate_period : 8u;
     sa.donate_target = std::max<uint32_t>(16, sa.nwaves / 4);
     const size_t nslots = size_t(sa.qcap) * NSUBQ;
     int rc;
@@ -2205,6 +2210,35 @@ int Engine::count_pending(std::vector<uint64_t> &per_job)
             if (root < C.hdesc.size() && C.hdesc[root].job < per_job.size()) per_job[C.hdesc[root].job]++;
         }
     }
+    return KP_OK;
+}
+
+int Engine::time_class(int c, const SearchOpts &o, float *ms)
+{
+    ClassState &C = cls[c];
+    const size_t saved_done = C.todo_done;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    struct EvGuard {
+        hipEvent_t a, b;
+        ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } eg{a, b};
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipEventRecord(a, C.stream));
+    int rc = launch_class_w(*this, c, o);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(b, C.stream));
+    HIPCHK(hipEventSynchronize(b));
+    C.launched = false;
+    C.todo_done = saved_done;
+    Sched hs{};
+    HIPCHK(hipMemcpy(&hs, C.sched.p, sizeof(Sched), hipMemcpyDeviceToHost));
+    if (hs.error) {
+        set_error("search kernel (class W=%d) reported error %d", kClassW[c], hs.error);
+        return hs.error;
+    }
+    HIPCHK(hipEventElapsedTime(ms, a, b));
     return KP_OK;
 }
 
@@ -2499,6 +2533,103 @@ int dev_run(kp_prepared *h, kp_result *res)
     res->search_ms = sms;
     res->wall_ms = res->preprocess_ms + dms + sms;
     return KP_OK;
+}
+
+// Coordinate descent over the launch settings of each size class, timing the
+// class alone on this prepared workload; the best settings become the
+// process-wide ones.
+int dev_autotune(kp_prepared *h, kp_tune *out)
+{
+    std::memset(out, 0, sizeof(*out));
+    HIPCHK(hipSetDevice(h->E.device));
+    Engine &E = h->E;
+    int rc;
+    if ((rc = E.reserve_jobs(1))) return rc;
+    E.set_job(0, h->gv, h->p.k, h->p.q);
+    if ((rc = E.sync_jobs())) return rc;
+    JobIn j{0, h->gv, h->p.k, h->p.q, 0, 0};
+    root_window(h->gv.pn, h->p, j.pos_begin, j.pos_end);
+    E.reset_tables();
+    if ((rc = E.zero_acc(0, 1))) return rc;
+    std::vector<JobStats> st;
+    auto alloc = [&E](size_t, uint64_t words) -> uint64_t * {
+        auto *b = new (std::nothrow) DevBuf();
+        if (!b) return nullptr;
+        if (b->ensure(8 * words) || hipMemsetAsync(b->p, 0, 8 * words, E.stream) != hipSuccess) {
+            delete b;
+            return nullptr;
+        }
+        E.segs.push_back(b);
+        return b->as<uint64_t>();
+    };
+    if ((rc = E.stage({j}, st, alloc))) return rc;
+    kp_params P = h->p;
+    P.pause_every = 0;
+    P.flags &= ~KP_FLAG_LIST;
+    int orc = KP_OK;
+    SearchOpts o = search_opts(E, P, E.device, orc);
+    if (orc) return orc;
+    if ((rc = E.list_used.ensure(8))) return rc;
+    for (int c = 0; c < kNumClass; c++) {
+        if (E.cls[c].htodo.empty()) continue;
+        ClassTune best = g_tune[c];
+        float t_best = 0, t = 0;
+        if ((rc = E.time_class(c, o, &t_best))) return rc;   // warm-up
+        if ((rc = E.time_class(c, o, &t_best))) return rc;
+        auto trial = [&](ClassTune cand) -> int {
+            g_tune[c] = cand;
+            int r = E.time_class(c, o, &t);
+            g_tune[c] = best;
+            if (r) return r;
+            if (t < 0.97f * t_best) {
+                best = cand;
+                t_best = t;
+                g_tune[c] = best;
+            }
+            return KP_OK;
+        };
+        const int nbmax = std::max(1, E.cls[c].blocks_per_cu);
+        for (int nb = nbmax - 1; nb >= 1; nb--) {
+            ClassTune x = best;
+            x.blocks_per_cu = nb;
+            if ((rc = trial(x))) return rc;
+        }
+        for (uint32_t dp : {4u, 16u, 32u}) {
+            ClassTune x = best;
+            x.donate_period = dp;
+            if ((rc = trial(x))) return rc;
+        }
+        for (uint32_t rf : {8u, 32u, 64u}) {
+            ClassTune x = best;
+            x.ring_frames = rf;
+            if ((rc = trial(x))) return rc;
+        }
+        for (uint32_t qs : {64u, 2048u}) {
+            ClassTune x = best;
+            x.queue_slots = qs;
+            if ((rc = trial(x))) return rc;
+        }
+        g_tune[c] = best;
+        out->blocks_per_cu[c] = best.blocks_per_cu;
+        out->donate_period[c] = best.donate_period;
+        out->ring_frames[c] = best.ring_frames;
+        out->queue_slots[c] = best.queue_slots;
+        out->class_ms[c] = t_best;
+    }
+    E.reset_tables();
+    return KP_OK;
+}
+
+void dev_set_tune(const kp_tune *t)
+{
+    for (int c = 0; c < kNumClass; c++) {
+        g_tune[c] = ClassTune{};
+        if (!t) continue;
+        g_tune[c].blocks_per_cu = t->blocks_per_cu[c];
+        g_tune[c].donate_period = t->donate_period[c];
+        g_tune[c].ring_frames = t->ring_frames[c];
+        g_tune[c].queue_slots = t->queue_slots[c];
+    }
 }
 
 // Many graphs in one pool: host preprocessing in parallel over the graphs,

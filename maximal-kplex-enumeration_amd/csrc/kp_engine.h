@@ -24,6 +24,16 @@ constexpr int kNumClass = 7;
 extern const int kClassW[kNumClass];   // bitset words per size class
 int class_of(uint32_t n);              // -1 when n > 64 * 16
 
+// Launch settings of one size class (0 = the built-in default); set by
+// kp_set_tune / kp_autotune and read by every engine at launch.
+struct ClassTune {
+    int blocks_per_cu = 0;       // <= the occupancy limit
+    uint32_t donate_period = 0;  // frames between donation checks (default 8)
+    uint32_t ring_frames = 0;    // LDS ring frames per wave (default: 8 KiB)
+    uint32_t queue_slots = 0;    // frame-queue slots per shard
+};
+extern ClassTune g_tune[kNumClass];
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -138,6 +148,9 @@ struct Engine {
     // allocates the pinned, device-mapped pause word that SearchOpts::host_pause polls
     int enable_host_pause();
     bool epoch_done();                 // every launched class has finished (does not block)
+    // launches class c alone over its whole start list and resumed frames and
+    // returns the kernel time; the class's tables are left as they were
+    int time_class(int c, const SearchOpts &o, float *ms);
 };
 
 int dev_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r);

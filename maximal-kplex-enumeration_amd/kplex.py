@@ -51,6 +51,14 @@ class KpResult(ctypes.Structure):
                 ("nlisted", c_uint64), ("list_off", POINTER(c_uint64)), ("list_ids", POINTER(c_uint32))]
 
 
+class KpTune(ctypes.Structure):
+    _fields_ = [("blocks_per_cu", ctypes.c_int32 * 7), ("donate_period", c_uint32 * 7),
+                ("ring_frames", c_uint32 * 7), ("queue_slots", c_uint32 * 7), ("class_ms", c_double * 7)]
+
+
+CLASS_WORDS = (1, 2, 3, 4, 6, 8, 16)
+
+
 class KpServerConfig(ctypes.Structure):
     _fields_ = [("max_inflight", c_size_t), ("mem_budget_bytes", c_uint64), ("slice_ms", c_double)]
 
@@ -68,6 +76,8 @@ SIGNATURES = {
     "kp_run": (c_int, [c_void_p, POINTER(KpResult)]),
     "kp_prepared_bytes": (c_uint64, [c_void_p]),
     "kp_prepared_free": (None, [c_void_p]),
+    "kp_autotune": (c_int, [c_void_p, POINTER(KpTune)]),
+    "kp_set_tune": (None, [POINTER(KpTune)]),
     "kp_enumerate_batch": (c_int, [POINTER(POINTER(KpGraph)), POINTER(KpParams), c_size_t, POINTER(KpResult)]),
     "kp_server_create": (c_int, [POINTER(KpServerConfig), POINTER(c_void_p)]),
     "kp_server_add_graph": (c_int, [c_void_p, POINTER(KpGraph), POINTER(c_uint32)]),
@@ -309,6 +319,15 @@ class Prepared:
     def bytes(self):
         return self.lib.kp_prepared_bytes(self.h)
 
+    def autotune(self):
+        """kp_autotune: returns {class words W: settings} for the classes present."""
+        t = KpTune()
+        _check(self.lib.kp_autotune(self.h, ctypes.byref(t)), "kp_autotune")
+        return {CLASS_WORDS[c]: dict(blocks_per_cu=t.blocks_per_cu[c], donate_period=t.donate_period[c],
+                                     ring_frames=t.ring_frames[c], queue_slots=t.queue_slots[c],
+                                     class_ms=t.class_ms[c])
+                for c in range(7) if t.class_ms[c] > 0}
+
     def close(self):
         if self.h:
             self.lib.kp_prepared_free(self.h)
@@ -319,6 +338,23 @@ class Prepared:
             self.close()
         except Exception:
             pass
+
+
+def set_tune(settings=None):
+    """kp_set_tune: settings = {class words W: dict(blocks_per_cu=, donate_period=, ring_frames=,
+    queue_slots=)} or None for the defaults."""
+    lib = load()
+    if settings is None:
+        lib.kp_set_tune(None)
+        return
+    t = KpTune()
+    for w, d in settings.items():
+        c = CLASS_WORDS.index(w)
+        t.blocks_per_cu[c] = d.get("blocks_per_cu", 0)
+        t.donate_period[c] = d.get("donate_period", 0)
+        t.ring_frames[c] = d.get("ring_frames", 0)
+        t.queue_slots[c] = d.get("queue_slots", 0)
+    lib.kp_set_tune(ctypes.byref(t))
 
 
 class Server:

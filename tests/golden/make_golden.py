@@ -47,6 +47,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--with-large", action="store_true",
                     help="also the Chung-Lu 75k root windows [4000,6000) and [0,9000) (minutes of oracle time)")
+    ap.add_argument("--with-spill", action="store_true",
+                    help="also the Chung-Lu 320k (truss) root window [0,800): 1.96e9 tasks (minutes of oracle time)")
     args = ap.parse_args()
     gold = {}
     if os.path.exists(OUT):
@@ -70,6 +72,9 @@ def main():
         og = from_graph(g)
         gold["cl75k_4000_6000"] = run(og, k, q, tr, window=(4000, 6000))
         gold["cl75k_0_9000"] = run(og, k, q, tr, window=(0, 9000))
+    if args.with_spill:
+        g, k, q, _, tr = kplex.workload_cl320k()
+        gold["cl320k_0_800"] = run(from_graph(g), k, q, tr, window=(0, 800))
     with open(OUT, "w") as f:
         json.dump(gold, f, indent=1)
     print(json.dumps({n: {x: v[x] for x in ("count", "hash", "tasks", "oracle_s")} for n, v in gold.items()}, indent=1))

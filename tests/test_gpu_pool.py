@@ -136,3 +136,22 @@ def test_batch_with_pauses():
         same(r, b)
     assert rs[0]["pauses"] > 0 and rs[0]["spilled"] > 0
     same(rs[-2], oracle(*mid_graph()))
+
+
+def test_autotune_keeps_results():
+    g, k, q = mid_graph()
+    say("autotune")
+    p = kplex.Prepared(g, k, q, time_limit_s=LIMIT)
+    base = p.run()
+    try:
+        tune = p.autotune()
+        assert tune and all(v["class_ms"] > 0 for v in tune.values())
+        same(p.run(), base)
+        # a batch after tuning uses the same settings and still agrees
+        rs = kplex.enumerate_batch([g, g], [(k, q), (2, 12)])
+        same(rs[0], base)
+        kplex.set_tune({1: dict(blocks_per_cu=1, donate_period=32, ring_frames=8, queue_slots=16)})
+        same(p.run(), base)
+    finally:
+        kplex.set_tune(None)
+        p.close()
