@@ -27,8 +27,8 @@ WORKLOADS = {
     "cl75k-full": ("chung-lu n=75000 gamma=2.1 maxdeg=3000 ~508k edges + 300 communities 35-60 @0.9, k=4 q=30",
                    "workload_cl75k", None),
     "er2000": ("G(2000,0.01) + 20 planted groups of 40, k=3 q=20", "workload_er2000", None),
-    "cl320k": ("chung-lu n=320000 gamma=2.0 ~1.05M edges + 1000 communities 30-80 @0.85, k=5 q=30, truss",
-               "workload_cl320k", None),
+    "cl320k": ("chung-lu n=320000 gamma=2.0 ~1.05M edges + 1000 communities 30-80 @0.85, k=5 q=30, truss; "
+               "roots at degeneracy positions [0,800) of 50565 (1.96e9 tasks)", "workload_cl320k", (0, 800)),
     "soc-epinions": ("reference dataset soc-epinions (README.md:142), k=4 q=30", None, None),
     "email-euall": ("reference dataset email-euall (README.md:144), k=3 q=12", None, None),
 }
@@ -219,6 +219,35 @@ def served_vs_plain(duration_s, rate, time_limit, seed=7):
             "preload_s": preload_s, "identical": True}
 
 
+def spill_line(time_limit, pause_every=200_000_000, queue_capacity=4096, thres=0.25):
+    """The large workload with truss (Chung-Lu 320k, k=5 q=30), roots
+    [0,800): once straight through, once with every launch paused after
+    `pause_every` frames and all but thres * queue_capacity frames per class
+    spilled to pinned host memory and refilled.  Results must match."""
+    import kplex
+    desc, fn, window = WORKLOADS["cl320k"]
+    g, k, q, _, tr = getattr(kplex, fn)()
+    log(f"[bench] spill: {desc}")
+    p = kplex.Prepared(g, k, q, 0.25, tr, time_limit_s=time_limit, roots=window)
+    p.run()
+    plain = p.run()
+    p.close()
+    p = kplex.Prepared(g, k, q, thres, tr, time_limit_s=time_limit, roots=window, queue_capacity=queue_capacity,
+                       pause_every=pause_every)
+    sp = p.run()
+    p.close()
+    for x in ("count", "hash", "tasks", "valid_roots"):
+        assert sp[x] == plain[x], ("spilled result differs", x)
+    log(f"[bench] spill: straight {plain['search_ms']:.0f} ms, with {sp['pauses']} pauses and {sp['spilled']} "
+        f"frames spilled {sp['search_ms']:.0f} ms")
+    return {"workload": desc, "count": plain["count"], "hash": plain["hash"], "tasks": plain["tasks"],
+            "straight_search_ms": plain["search_ms"], "spill_search_ms": sp["search_ms"],
+            "pause_every_frames": pause_every, "device_frames_per_class": int(thres * queue_capacity),
+            "pauses": sp["pauses"], "spilled_frames": sp["spilled"],
+            "plexes_per_s_straight": plain["count"] / (plain["search_ms"] / 1e3),
+            "plexes_per_s_spill": sp["count"] / (sp["search_ms"] / 1e3), "identical": True}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +263,7 @@ def main():
                     help="arrival window of the served stream at --served-rate queries/s (0 = skip)")
     ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--no-autotune", action="store_true", help="time with the built-in launch settings")
+    ap.add_argument("--no-spill-line", action="store_true", help="skip the pause/spill line on the 320k workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -313,6 +343,8 @@ def main():
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline (target {args.cpu_target_s} s)")
         out["cpu_baseline"] = cpu_baseline(graph, k, q, truss, args.cpu_target_s, window)
+    if not args.no_spill_line:
+        out["spill"] = spill_line(args.time_limit)
     if args.batch_graphs > 0:
         out["batched"] = batched_vs_plain(args.batch_graphs, args.time_limit)
     if args.served_seconds > 0:

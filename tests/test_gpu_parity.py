@@ -189,3 +189,19 @@ def test_time_limit_is_reported():
     # the library is usable afterwards
     r = gpu(g, k, q, tr, name="after-timeout")
     assert r["count"] == 20
+
+
+def test_cl320k_window_with_spill_against_golden(gold):
+    # the Chung-Lu 320k workload with truss, roots [0,800): 1.96e9 tasks; each
+    # launch pauses every 2e8 frames and at most 1,024 frames per size class
+    # stay on the device, the rest go to pinned host memory and come back
+    key = "cl320k_0_800"
+    if key not in gold:
+        pytest.skip(f"no recorded oracle result for {key}")
+    g, k, q, _, tr = kplex.workload_cl320k()
+    say(f"{key} n={g.n} m={g.m} with pauses and spill")
+    r = kplex.enumerate_plexes(g, k, q, truss=tr, time_limit_s=LIMIT, roots=(0, 800), thres=0.25,
+                               queue_capacity=4096, pause_every=200_000_000)
+    same(r, gold[key], ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l"))
+    assert r["tasks"] > 1_000_000_000
+    assert r["pauses"] >= 5 and r["spilled"] > 0
