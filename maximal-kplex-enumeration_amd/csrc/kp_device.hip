@@ -597,9 +597,50 @@ struct Worker {
         cold->check_every =
             a.pause_frames ? uint32_t(max(uint64_t(32), min(uint64_t(1024), a.pause_frames / max(1u, a.nwaves)))) : 1024u;
     }
+    // Re-asserts that the wave-uniform state is uniform.  The optimizer merges
+    // "if (uniform) { if (lane == 0) <memory op>; <uniform update> }" into one
+    // branch on a lane-dependent condition, and the join then looks
+    // lane-dependent to the divergence analysis; readfirstlane at the top of
+    // the frame loop cuts that, so the loop body keeps uniform values in
+    // scalar registers (a readfirstlane of a scalar register is a plain copy).
+    __device__ __forceinline__ void launder()
+    {
+        rb = rfl(rb);
+        rn = rfl(rn);
+        bb = rfl(bb);
+        bn = rfl(bn);
+        cur = rfl(cur);
+        n = rfl(n);
+        h = rfl(h);
+        l = rfl(l);
+        k = int(rfl(uint32_t(k)));
+        q = int(rfl(uint32_t(q)));
+        LA = reinterpret_cast<const uint64_t *>(rfl64(reinterpret_cast<uint64_t>(LA)));
+        cnt = rfl64(cnt);
+        ha = rfl64(ha);
+        hb = rfl64(hb);
+        tasks = rfl64(tasks);
+        spills = rfl(spills);
+        err = int(rfl(uint32_t(err)));
+    }
+    static __device__ __forceinline__ void launder(Frame<W> &f)
+    {
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            f.P[w] = rfl64(f.P[w]);
+            f.C[w] = rfl64(f.C[w]);
+            f.D[w] = rfl64(f.D[w]);
+            f.X[w] = rfl64(f.X[w]);
+        }
+        f.root = rfl(f.root);
+        f.type = rfl(f.type);
+        f.r = rfl(f.r);
+    }
+    static __device__ __forceinline__ bool uflag(bool b) { return rfl(uint32_t(b)) != 0; }
+
     __device__ __forceinline__ bool past_deadline() const
     {
-        return a.limit_ticks && wall_clock64() - cold->t_start > a.limit_ticks;
+        return a.limit_ticks && wall_clock64() - rfl64(cold->t_start) > a.limit_ticks;
     }
     __device__ __forceinline__ void set_error(int code)
     {
@@ -719,7 +760,7 @@ struct Worker {
     }
 
     // ---- sharded MPMC frame queue ----
-    __device__ __forceinline__ uint32_t home_shard() const { return (blockIdx.x * 4u + (threadIdx.x >> 6)) % NSUBQ; }
+    __device__ __forceinline__ uint32_t home_shard() const { return rfl((blockIdx.x * 4u + (threadIdx.x >> 6)) % NSUBQ); }
 
     // Reserves a free slot in some shard (home shard first).  Returns the
     // global slot index, or ~0 when every shard is full.
@@ -820,14 +861,16 @@ struct Worker {
     // task bytes of the current root so far go to Cold::alg
     __device__ void close_alg()
     {
-        cold->alg = cold->alg + (tasks - cold->task0) * cold->bpt;
+        cold->alg = rfl64(cold->alg) + (tasks - rfl64(cold->task0)) * rfl64(cold->bpt);
         cold->task0 = tasks;
     }
     // adds the accumulators to their job's totals (close_alg first)
     __device__ void flush()
     {
-        const uint32_t job = cold->cjob;
-        const uint64_t alg = cold->alg;
+        // LDS values go through readfirstlane: the compiler then knows they
+        // are wave-uniform and keeps what depends on them in scalar registers
+        const uint32_t job = rfl(cold->cjob);
+        const uint64_t alg = rfl64(cold->alg);
         if (job != 0xffffffffu && ln == 0 && (cnt | tasks)) {
             JobAcc *J = a.acc + job;
             atomicAdd(&J->count, (unsigned long long)cnt);
@@ -853,7 +896,7 @@ struct Worker {
         k = int(kq & 0xffffu);
         q = int(kq >> 16);
         close_alg();
-        if (job != cold->cjob) {
+        if (job != rfl(cold->cjob)) {
             flush();
             cold->cjob = job;
         }
@@ -904,7 +947,13 @@ struct Worker {
         for (int w = 0; w < W; w++) c += __popcll(adjR[j][w] & Sx[w]);
         return c;
     }
-    __device__ __forceinline__ bool has(const uint64_t *Sx, int j) const { return (Sx[j] >> ln) & 1; }
+    // lane predicate "vertex 64j+lane is in Sx": the uniform word is the lane
+    // mask itself (no per-lane shift)
+    static __device__ __forceinline__ bool has(const uint64_t *Sx, int j)
+    {
+        return __builtin_amdgcn_inverse_ballot_w64(Sx[j]);
+    }
+    static __device__ __forceinline__ bool inmask(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
     __device__ __forceinline__ bool subset_of_adj(int j, const uint64_t *Sx) const
     {
         uint64_t bad = 0;
@@ -923,11 +972,10 @@ struct Worker {
     // upperBound (:1002-1040): every P vertex keeps >= q-k neighbours in Sx
     __device__ __forceinline__ bool ub_ok(const uint64_t *P, const uint64_t *Sx) const
     {
-        bool fail = false;
+        uint64_t fail = 0;
 #pragma unroll
-        for (int j = 0; j < W; j++)
-            if (has(P, j) && int(deg_in(j, Sx)) + k < q) fail = true;
-        return __ballot(fail) == 0;
+        for (int j = 0; j < W; j++) fail |= P[j] & __ballot(int(deg_in(j, Sx)) + k < q);
+        return fail == 0;
     }
 
     // kplex-compatibility mask w.r.t. P (isKplex2/isKplex3 :568-598)
@@ -938,7 +986,7 @@ struct Worker {
 #pragma unroll
         for (int j = 0; j < W; j++) {
             dP[j] = deg_in(j, P);
-            Sat[j] = __ballot(has(P, j) && int(dP[j]) + k == int(ps));
+            Sat[j] = P[j] & __ballot(int(dP[j]) + k == int(ps));
         }
 #pragma unroll
         for (int j = 0; j < W; j++) OK[j] = __ballot(int(dP[j]) + k >= int(ps) + 1 && subset_of_adj(j, Sat));
@@ -950,7 +998,7 @@ struct Worker {
     {
         uint64_t Sat[W];
 #pragma unroll
-        for (int j = 0; j < W; j++) Sat[j] = __ballot(has(Sx, j) && int(deg_in(j, Sx)) + k < int(s) + 1);
+        for (int j = 0; j < W; j++) Sat[j] = Sx[j] & __ballot(int(deg_in(j, Sx)) + k < int(s) + 1);
 #pragma unroll
         for (int t = 0; t < LW; t++) {
             if (64u * t >= l) return true;
@@ -1055,7 +1103,7 @@ struct Worker {
             uint32_t k2 = ~0u;
 #pragma unroll
             for (int j = 0; j < W; j++)
-                if (((C[j] & ~adjPP[j]) >> ln) & 1) k2 = min(k2, (dG[j] << 20) | (dP[j] << 10) | (64u * j + ln));
+                if (inmask(C[j] & ~adjPP[j])) k2 = min(k2, (dG[j] << 20) | (dP[j] << 10) | (64u * j + ln));
             k2 = rfl(wmin32(k2));
             if (k2 == ~0u) {   // unreachable while the degrees are consistent
                 set_error(KP_EINVAL);
@@ -1074,12 +1122,11 @@ struct Worker {
                 const uint32_t tot = ps + cs;
                 uint64_t Sat[W];
 #pragma unroll
-                for (int j = 0; j < W; j++) Sat[j] = __ballot(has(PC, j) && int(dG[j]) + k == int(tot));
-                bool ext = false;
+                for (int j = 0; j < W; j++) Sat[j] = PC[j] & __ballot(int(dG[j]) + k == int(tot));
+                uint64_t ext = 0;
 #pragma unroll
-                for (int j = 0; j < W; j++)
-                    if (has(X, j) && int(dG[j]) + k >= int(tot) + 1 && subset_of_adj(j, Sat)) ext = true;
-                if (__ballot(ext)) return false;
+                for (int j = 0; j < W; j++) ext |= X[j] & __ballot(int(dG[j]) + k >= int(tot) + 1 && subset_of_adj(j, Sat));
+                if (ext) return false;
                 if (maximal(PC, tot)) record(PC, tot);
                 return false;
             }
@@ -1292,10 +1339,18 @@ struct Worker {
     {
         bool active_self = false, hungry_self = false, have = false, paused = false;
         uint32_t since_donate = 0, backoff = 1;
-        uint32_t to_check = cold->check_every;   // frames left until the next check
-        const uint32_t gw = blockIdx.x * 4u + (threadIdx.x >> 6);
+        uint32_t to_check = rfl(cold->check_every);   // frames left until the next check
+        const uint32_t gw = rfl(blockIdx.x * 4u + (threadIdx.x >> 6));
         Frame<W> f, nf;
         for (;;) {
+            launder();
+            launder(f);
+            have = uflag(have);
+            active_self = uflag(active_self);
+            hungry_self = uflag(hungry_self);
+            since_donate = rfl(since_donate);
+            to_check = rfl(to_check);
+            backoff = rfl(backoff);
             if (!have) {
                 if (depth() == 0) {
                     if (active_self) {
@@ -1435,7 +1490,7 @@ struct Worker {
                 }
             }
             if (--to_check == 0) {
-                const uint32_t ce = cold->check_every;
+                const uint32_t ce = rfl(cold->check_every);
                 to_check = ce;
                 cold->frames = cold->frames + ce;
                 uint32_t stop = 0;
@@ -1462,8 +1517,8 @@ struct Worker {
         }
         close_alg();
         flush();
-        const uint64_t busy_t = cold->busy_t, idle_t = cold->idle_t, pops = cold->pops;
-        const uint64_t frames = cold->frames + (cold->check_every - to_check);
+        const uint64_t busy_t = rfl64(cold->busy_t), idle_t = rfl64(cold->idle_t), pops = rfl64(cold->pops);
+        const uint64_t frames = rfl64(cold->frames) + (rfl(cold->check_every) - to_check);
         uint32_t last = 0;
         if (ln == 0) {
             atomicAdd((unsigned long long *)&S->busy_ticks, (unsigned long long)busy_t);
@@ -1500,7 +1555,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1 ? 4 
     extern __shared__ uint64_t lds[];
     __shared__ Cold s_cold[4];
     constexpr int FW = FrameLayout<W>::FW;
-    const int wv = threadIdx.x >> 6;
+    // the wave index is uniform within a wave; readfirstlane tells the
+    // compiler, so that everything derived from it (ring, stack, LDS state,
+    // the frame loop's control flow) stays scalar
+    const int wv = int(rfl(threadIdx.x >> 6));
     const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + wv;
     if (gw >= a.nwaves) return;
     uint64_t *ring = lds + size_t(wv) * a.ring_cap * FW;
