@@ -638,13 +638,23 @@ struct Worker {
     }
     static __device__ __forceinline__ bool uflag(bool b) { return rfl(uint32_t(b)) != 0; }
 
+    // The branch to lane 0 starts with an empty volatile asm: it keeps the
+    // optimizer from folding the lane test into an enclosing uniform branch
+    // (which would make the join, and the wave-uniform state merged there,
+    // look lane-dependent).
+    __device__ __forceinline__ bool lane0() const
+    {
+        asm volatile("");
+        return ln == 0;
+    }
+
     __device__ __forceinline__ bool past_deadline() const
     {
         return a.limit_ticks && wall_clock64() - rfl64(cold->t_start) > a.limit_ticks;
     }
     __device__ __forceinline__ void set_error(int code)
     {
-        if (ln == 0) atomicCAS(&S->error, 0, code);
+        if (lane0()) atomicCAS(&S->error, 0, code);
         err = code;
     }
 
@@ -652,7 +662,7 @@ struct Worker {
     __device__ __forceinline__ void store_slot(uint64_t *dst, uint64_t meta, const uint64_t *P, const uint64_t *C,
                                                const uint64_t *X, const uint64_t *D, bool withD)
     {
-        if (ln == 0) {
+        if (lane0()) {
             dst[0] = meta;
 #pragma unroll
             for (int w = 0; w < W; w++) {
@@ -667,7 +677,7 @@ struct Worker {
     {
         uint64_t meta = 0;
         uint64_t t[4 * W];
-        if (ln == 0) {
+        if (lane0()) {
             meta = src[0];
 #pragma unroll
             for (int i = 0; i < 3 * W; i++) t[i] = src[1 + i];
@@ -676,7 +686,7 @@ struct Worker {
         f.root = uint32_t(meta);
         f.type = uint32_t(meta >> 32) & 0xff;
         f.r = uint32_t(meta >> 40) & 0xff;
-        if (f.type == FT_KSEARCH && ln == 0) {
+        if (f.type == FT_KSEARCH && lane0()) {
 #pragma unroll
             for (int i = 0; i < W; i++) t[3 * W + i] = src[1 + 3 * W + i];
         }
@@ -694,7 +704,7 @@ struct Worker {
     }
     __device__ __forceinline__ void copy_slot(uint64_t *dst, const uint64_t *src)
     {
-        if (ln == 0) {
+        if (lane0()) {
 #pragma unroll
             for (int i = 0; i < FW; i++) dst[i] = src[i];
         }
@@ -768,7 +778,7 @@ struct Worker {
     {
         uint32_t ok = 0, shard = 0;
         uint64_t t = 0;
-        if (ln == 0) {
+        if (lane0()) {
             const uint32_t home = home_shard();
             for (uint32_t p = 0; p < NSUBQ && !ok; p++) {
                 const uint32_t j = (home + p) % NSUBQ;
@@ -787,7 +797,7 @@ struct Worker {
         t = rfl64(t);
         const size_t slot = size_t(shard) * a.qcap + uint32_t(t % a.qcap);
         uint32_t bad = 0;
-        if (ln == 0) {
+        if (lane0()) {
             uint32_t spins = 0;
             while (__hip_atomic_load(&a.qseq[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != t) {
                 __builtin_amdgcn_s_sleep(1);
@@ -805,7 +815,7 @@ struct Worker {
     __device__ void q_publish(size_t slot, uint32_t shard, uint64_t t, const Frame<W> &f)
     {
         store_slot(a.qframes + slot * FW, make_meta(f.root, f.type, f.r), f.P, f.C, f.X, f.D, f.type == FT_KSEARCH);
-        if (ln == 0) {
+        if (lane0()) {
             __hip_atomic_store(&a.qseq[slot], t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_add(&a.subq[shard].count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_add(&S->q_count, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
@@ -816,7 +826,7 @@ struct Worker {
     {
         uint32_t ok = 0, shard = 0;
         uint64_t hd = 0;
-        if (ln == 0) {
+        if (lane0()) {
             const uint32_t home = home_shard();
             for (uint32_t p = 0; p < NSUBQ && !ok; p++) {
                 const uint32_t j = (home + p) % NSUBQ;
@@ -837,7 +847,7 @@ struct Worker {
         hd = rfl64(hd);
         const size_t slot = size_t(shard) * a.qcap + uint32_t(hd % a.qcap);
         uint32_t bad = 0;
-        if (ln == 0) {
+        if (lane0()) {
             uint32_t spins = 0;
             while (__hip_atomic_load(&a.qseq[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != hd + 1) {
                 __builtin_amdgcn_s_sleep(1);
@@ -849,7 +859,7 @@ struct Worker {
             return false;
         }
         load_slot(a.qframes + slot * FW, f);
-        if (ln == 0) {
+        if (lane0()) {
             __hip_atomic_store(&a.qseq[slot], hd + a.qcap, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             atomicSub(&a.subq[shard].size, 1u);
             __hip_atomic_fetch_sub(&S->q_count, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
@@ -871,7 +881,7 @@ struct Worker {
         // are wave-uniform and keeps what depends on them in scalar registers
         const uint32_t job = rfl(cold->cjob);
         const uint64_t alg = rfl64(cold->alg);
-        if (job != 0xffffffffu && ln == 0 && (cnt | tasks)) {
+        if (job != 0xffffffffu && (cnt | tasks) && lane0()) {
             JobAcc *J = a.acc + job;
             atomicAdd(&J->count, (unsigned long long)cnt);
             atomicAdd(&J->ha, (unsigned long long)ha);
@@ -1046,13 +1056,13 @@ struct Worker {
         hb += mix64(sb + uint64_t(s) * kp::HASH_LB);
         if (a.list) {
             uint64_t base = 0;
-            if (ln == 0) base = atomicAdd((unsigned long long *)a.list_used, (unsigned long long)(s + 2));
+            if (lane0()) base = atomicAdd((unsigned long long *)a.list_used, (unsigned long long)(s + 2));
             base = rfl64(base);
             if (base + s + 2 > a.list_cap) {
                 set_error(KP_EOVERFLOW);
                 return;
             }
-            if (ln == 0) {
+            if (lane0()) {
                 a.list[base] = s;
                 a.list[base + 1] = cold->cjob;
             }
@@ -1288,7 +1298,7 @@ struct Worker {
     __device__ bool pause_requested(uint32_t new_frames)
     {
         uint32_t pz = 0;
-        if (ln == 0) {
+        if (lane0()) {
             pz = __hip_atomic_load(&S->pause, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (!pz && a.host_pause &&
                 __hip_atomic_load(a.host_pause, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0)
@@ -1309,7 +1319,7 @@ struct Worker {
         const uint32_t m = depth() + (have ? 1u : 0u);
         if (m == 0) return;
         uint32_t base = 0;
-        if (ln == 0) base = atomicAdd(&S->dump_used, m);
+        if (lane0()) base = atomicAdd(&S->dump_used, m);
         base = rfl(base);
         if (uint64_t(base) + m > a.dump_cap) {
             set_error(KP_EOVERFLOW);
@@ -1357,14 +1367,14 @@ struct Worker {
                         const uint64_t now = wall_clock64();
                         cold->busy_t = cold->busy_t + (now - cold->t_mark);
                         cold->t_mark = now;
-                        if (ln == 0) __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                        if (lane0()) __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                         active_self = false;
                     }
                     if (err) break;
                     // hints only (relaxed): claims below are atomic read-modify-writes
                     bool got = false;
                     uint32_t qc = 0, nr = 0, ns = 0;
-                    if (ln == 0) {
+                    if (lane0()) {
                         qc = __hip_atomic_load(&S->q_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         ns = __hip_atomic_load(&S->next_seed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         nr = __hip_atomic_load(&S->next_root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1373,19 +1383,19 @@ struct Worker {
                     ns = rfl(ns);
                     nr = rfl(nr);
                     if (qc > 0) {
-                        if (ln == 0) __hip_atomic_fetch_add(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                        if (lane0()) __hip_atomic_fetch_add(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                         got = q_pop(f);
                         if (got) {
                             active_self = true;
                             have = true;
-                        } else if (ln == 0) {
+                        } else if (lane0()) {
                             __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                         }
                         if (err) break;
                     }
                     if (!got && ns < a.nseed) {
                         uint32_t si = 0xffffffffu;
-                        if (ln == 0) {
+                        if (lane0()) {
                             __hip_atomic_fetch_add(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                             si = __hip_atomic_fetch_add(&S->next_seed, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                             if (si >= a.nseed)
@@ -1401,7 +1411,7 @@ struct Worker {
                     }
                     if (!got && nr < a.ntodo) {
                         uint32_t rr = 0xffffffffu;
-                        if (ln == 0) {
+                        if (lane0()) {
                             __hip_atomic_fetch_add(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                             rr = __hip_atomic_fetch_add(&S->next_root, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                             if (rr >= a.ntodo)
@@ -1416,11 +1426,11 @@ struct Worker {
                     }
                     if (!got) {
                         if (!hungry_self) {
-                            if (ln == 0) atomicAdd(&S->hungry, 1u);
+                            if (lane0()) atomicAdd(&S->hungry, 1u);
                             hungry_self = true;
                         }
                         uint32_t done = 0, stop = 0;
-                        if (ln == 0) {
+                        if (lane0()) {
                             // a relaxed look first; the seq_cst reads (in this
                             // order) confirm termination
                             const bool maybe =
@@ -1451,7 +1461,7 @@ struct Worker {
                         continue;
                     }
                     if (hungry_self) {
-                        if (ln == 0) atomicSub(&S->hungry, 1u);
+                        if (lane0()) atomicSub(&S->hungry, 1u);
                         hungry_self = false;
                     }
                     backoff = 1;
@@ -1470,7 +1480,7 @@ struct Worker {
                 const uint32_t avail = depth() + (have ? 1u : 0u);
                 if (avail >= 2 && depth() >= 1) {
                     uint32_t want = 0;
-                    if (ln == 0) {
+                    if (lane0()) {
                         const uint32_t hg = __hip_atomic_load(&S->hungry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         const uint32_t qc = __hip_atomic_load(&S->q_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         want = hg > qc ? min(hg - qc, 4u) : 0u;
@@ -1484,7 +1494,7 @@ struct Worker {
                         Frame<W> b;
                         pop_bottom(b);
                         q_publish(slot, shard, t, b);
-                        if (ln == 0) atomicAdd((unsigned long long *)&S->donated, 1ull);
+                        if (lane0()) atomicAdd((unsigned long long *)&S->donated, 1ull);
                     }
                     if (err) break;
                 }
@@ -1494,7 +1504,7 @@ struct Worker {
                 to_check = ce;
                 cold->frames = cold->frames + ce;
                 uint32_t stop = 0;
-                if (ln == 0) {
+                if (lane0()) {
                     stop = __hip_atomic_load(&S->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
                     if (past_deadline()) stop = 2;
                 }
@@ -1508,8 +1518,8 @@ struct Worker {
             }
         }
         if (paused && !err) dump_frames(have, f);
-        if (active_self && ln == 0) __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-        if (hungry_self && ln == 0) atomicSub(&S->hungry, 1u);
+        if (active_self && lane0()) __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+        if (hungry_self && lane0()) atomicSub(&S->hungry, 1u);
         {
             const uint64_t now = wall_clock64();
             if (active_self) cold->busy_t = cold->busy_t + (now - cold->t_mark);
@@ -1520,7 +1530,7 @@ struct Worker {
         const uint64_t busy_t = rfl64(cold->busy_t), idle_t = rfl64(cold->idle_t), pops = rfl64(cold->pops);
         const uint64_t frames = rfl64(cold->frames) + (rfl(cold->check_every) - to_check);
         uint32_t last = 0;
-        if (ln == 0) {
+        if (lane0()) {
             atomicAdd((unsigned long long *)&S->busy_ticks, (unsigned long long)busy_t);
             atomicAdd((unsigned long long *)&S->idle_ticks, (unsigned long long)idle_t);
             atomicAdd((unsigned long long *)&S->frames, (unsigned long long)frames);
@@ -1532,7 +1542,7 @@ struct Worker {
         // wave has exited, so nothing publishes or pops concurrently.
         if (rfl(last)) {
             uint32_t pz = 0, qc = 0;
-            if (ln == 0) {
+            if (lane0()) {
                 pz = __hip_atomic_load(&S->pause, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                 qc = __hip_atomic_load(&S->q_count, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
             }
