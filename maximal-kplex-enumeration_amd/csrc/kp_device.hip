@@ -2039,7 +2039,7 @@ int Engine::stage(const std::vector<JobIn> &jv, std::vector<JobStats> &st,
 namespace {
 
 template <int W, int LW>
-int launch_class(Engine &E, int c, const SearchOpts &o)
+int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
 {
     ClassState &C = E.cls[c];
     constexpr int FW = FrameLayout<W>::FW;
@@ -2055,7 +2055,10 @@ int launch_class(Engine &E, int c, const SearchOpts &o)
     int nb = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, dyn));
     if (nb < 1) nb = 1;
+    // Untuned, a class that is not the epoch's largest gets one block per CU,
+    // so that it can run beside the largest one instead of after it.
     if (T.blocks_per_cu > 0) nb = std::min(nb, T.blocks_per_cu);
+    else if (secondary) nb = 1;
     const uint32_t nwg = uint32_t(nb) * uint32_t(E.cus);
     SearchArgs sa{};
     sa.nwaves = nwg * 4;
@@ -2126,16 +2129,16 @@ int launch_class(Engine &E, int c, const SearchOpts &o)
     return KP_OK;
 }
 
-int launch_class_w(Engine &E, int c, const SearchOpts &o)
+int launch_class_w(Engine &E, int c, const SearchOpts &o, bool secondary = false)
 {
     switch (kClassW[c]) {
-    case 1: return launch_class<1, 2>(E, c, o);
-    case 2: return launch_class<2, 2>(E, c, o);
-    case 3: return launch_class<3, 2>(E, c, o);
-    case 4: return launch_class<4, 1>(E, c, o);
-    case 6: return launch_class<6, 0>(E, c, o);
-    case 8: return launch_class<8, 0>(E, c, o);
-    case 16: return launch_class<16, 0>(E, c, o);
+    case 1: return launch_class<1, 2>(E, c, o, secondary);
+    case 2: return launch_class<2, 2>(E, c, o, secondary);
+    case 3: return launch_class<3, 2>(E, c, o, secondary);
+    case 4: return launch_class<4, 1>(E, c, o, secondary);
+    case 6: return launch_class<6, 0>(E, c, o, secondary);
+    case 8: return launch_class<8, 0>(E, c, o, secondary);
+    case 16: return launch_class<16, 0>(E, c, o, secondary);
     }
     return KP_EINVAL;
 }
@@ -2232,7 +2235,7 @@ int Engine::launch_epoch(const SearchOpts &o)
     for (size_t oi = 0; oi < order.size() && !rc; oi++) {
         const int c = order[oi].second;
         if (hipStreamWaitEvent(cls[c].stream, ev, 0) != hipSuccess) rc = KP_EHIP;
-        else rc = launch_class_w(*this, c, o);
+        else rc = launch_class_w(*this, c, o, oi > 0);
         if (!rc && std::getenv("KP_SERIAL_CLASSES") && hipStreamSynchronize(cls[c].stream) != hipSuccess) rc = KP_EHIP;
     }
     (void)hipEventDestroy(ev);
