@@ -2055,8 +2055,9 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     int nb = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, dyn));
     if (nb < 1) nb = 1;
-    // Untuned, a class that is not the epoch's largest gets one block per CU,
-    // so that it can run beside the largest one instead of after it.
+    // Untuned, when the epoch's largest class is a wide one (two or more
+    // words, so at most two blocks per CU and VGPRs to spare), each other
+    // class gets one block per CU and runs beside it instead of after it.
     if (T.blocks_per_cu > 0) nb = std::min(nb, T.blocks_per_cu);
     else if (secondary) nb = 1;
     const uint32_t nwg = uint32_t(nb) * uint32_t(E.cus);
@@ -2235,7 +2236,7 @@ int Engine::launch_epoch(const SearchOpts &o)
     for (size_t oi = 0; oi < order.size() && !rc; oi++) {
         const int c = order[oi].second;
         if (hipStreamWaitEvent(cls[c].stream, ev, 0) != hipSuccess) rc = KP_EHIP;
-        else rc = launch_class_w(*this, c, o, oi > 0);
+        else rc = launch_class_w(*this, c, o, oi > 0 && order[0].second > 0);
         if (!rc && std::getenv("KP_SERIAL_CLASSES") && hipStreamSynchronize(cls[c].stream) != hipSuccess) rc = KP_EHIP;
     }
     (void)hipEventDestroy(ev);
