@@ -66,8 +66,10 @@ __device__ __forceinline__ uint64_t rfl64(uint64_t x)
     return (uint64_t(rfl(uint32_t(x >> 32))) << 32) | rfl(uint32_t(x));
 }
 // Wave64 reductions: four DPP steps (quad_perm [1,0,3,2], [2,3,0,1],
-// row_half_mirror, row_mirror) reduce each 16-lane row, then the four row
-// results are read out with v_readlane.  The result is wave-uniform.
+// row_half_mirror, row_mirror) leave every lane of a 16-lane row holding the
+// row's result; row_bcast:15 (rows 1 and 3 take the last lane of the row
+// before) and row_bcast:31 (rows 2 and 3 take lane 31) then carry it across
+// rows, so lane 63 holds the wave's result, read with one v_readlane.
 #define KP_DPP(v, ctrl) uint32_t(__builtin_amdgcn_update_dpp(0, int(v), ctrl, 0xF, 0xF, false))
 __device__ __forceinline__ uint32_t wmin32(uint32_t v)
 {
@@ -75,9 +77,11 @@ __device__ __forceinline__ uint32_t wmin32(uint32_t v)
     v = min(v, KP_DPP(v, 0x4E));
     v = min(v, KP_DPP(v, 0x141));
     v = min(v, KP_DPP(v, 0x140));
-    const uint32_t a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
-    const uint32_t c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
-    return min(min(a, b), min(c, d));
+    // the two cross-row steps as DPP-fused mins: rows outside row_mask keep
+    // their value (s_nop 1 covers the VALU-write -> DPP-read hazard)
+    asm volatile("s_nop 1\n\tv_min_u32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf" : "+v"(v));
+    asm volatile("s_nop 1\n\tv_min_u32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf" : "+v"(v));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 __device__ __forceinline__ uint64_t dpp64(uint64_t v, int ctrl_sel)
 {
