@@ -246,7 +246,12 @@ int admit(kp_server *s)
         Resident *res = nullptr;
         double pms = 0;
         int rc = get_resident(s, q, &res, &pms);
-        if (rc) return rc;
+        if (rc) {
+            // the query cannot run (its preprocessing failed): it leaves the
+            // queue and the error is returned once
+            s->pending.pop_front();
+            return rc;
+        }
         const uint32_t slot = s->free_slots.back();
         s->free_slots.pop_back();
         s->pending.pop_front();

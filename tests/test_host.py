@@ -144,3 +144,17 @@ def test_kp_enumerate_without_gpu_fails_loudly():
     with pytest.raises(kplex.KplexError) as e:
         kplex.enumerate_plexes(g, 2, 4)
     assert e.value.code in (kplex.KP_ENODEV, kplex.KP_EHIP)
+
+
+def test_tune_settings_without_device():
+    # the launch-settings table is host state: setting and clearing it needs no GPU
+    kplex.set_tune({1: dict(blocks_per_cu=1, donate_period=32), 16: dict(ring_frames=8)})
+    kplex.set_tune(None)
+
+
+def test_server_needs_a_device():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is visible")
+    with pytest.raises(kplex.KplexError) as e:
+        kplex.Server()
+    assert e.value.code == kplex.KP_ENODEV
