@@ -145,7 +145,7 @@ void kp_prepared_free(kp_prepared *h);
  * vertices, W = {1, 2, 3, 4, 6, 8, 16}; a 0 entry keeps the built-in default. */
 typedef struct kp_tune {
     int32_t  blocks_per_cu[7];  /* resident 256-thread blocks per CU (<= occupancy limit) */
-    uint32_t donate_period[7];  /* frames between donation checks (default 32) */
+    uint32_t donate_period[7];  /* frames between donation checks (default 256) */
     uint32_t ring_frames[7];    /* frames per wave in the LDS stack ring (default 8 KiB worth) */
     uint32_t queue_slots[7];    /* frame-queue slots per shard (default max(256, waves / 8)) */
     double   class_ms[7];       /* kp_autotune: best search time of the class alone */

@@ -2070,7 +2070,7 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     sa.qcap = T.queue_slots ? std::max<uint32_t>(16, T.queue_slots) : std::max<uint32_t>(256, sa.nwaves * 4 / NSUBQ);
     sa.ring_cap = ring;
     sa.back_cap = back;
-    sa.donate_period = T.donate_period ? T.donate_period : 8u;
+    sa.donate_period = T.donate_period ? T.donate_period : 256u;
     sa.donate_target = std::max<uint32_t>(16, sa.nwaves / 4);
     const size_t nslots = size_t(sa.qcap) * NSUBQ;
     int rc;
@@ -2670,7 +2670,7 @@ int dev_autotune(kp_prepared *h, kp_tune *out)
             x.blocks_per_cu = nb;
             if ((rc = trial(x))) return rc;
         }
-        for (uint32_t dp : {4u, 16u, 32u}) {
+        for (uint32_t dp : {32u, 64u, 128u, 512u, 1024u}) {
             ClassTune x = best;
             x.donate_period = dp;
             if ((rc = trial(x))) return rc;

@@ -28,7 +28,7 @@ int class_of(uint32_t n);              // -1 when n > 64 * 16
 // kp_set_tune / kp_autotune and read by every engine at launch.
 struct ClassTune {
     int blocks_per_cu = 0;       // <= the occupancy limit
-    uint32_t donate_period = 0;  // frames between donation checks (default 32)
+    uint32_t donate_period = 0;  // frames between donation checks (default 256)
     uint32_t ring_frames = 0;    // LDS ring frames per wave (default: 8 KiB)
     uint32_t queue_slots = 0;    // frame-queue slots per shard
 };
