@@ -819,10 +819,18 @@ struct Worker {
     __device__ void q_publish(size_t slot, uint32_t shard, uint64_t t, const Frame<W> &f)
     {
         store_slot(a.qframes + slot * FW, make_meta(f.root, f.type, f.r), f.P, f.C, f.X, f.D, f.type == FT_KSEARCH);
+        // One agent-scope release (an L2 write-back on MI355X, whose L2 is
+        // per XCD) orders the frame before the slot's sequence number; the
+        // counters are relaxed.  A consumer validates the slot with an
+        // acquire load of the sequence number.  The donor stays counted in
+        // S->active until after this, and its seq_cst decrement of
+        // S->active later orders the q_count increment for the termination
+        // test.
         if (lane0()) {
-            __hip_atomic_store(&a.qseq[slot], t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&a.subq[shard].count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&S->q_count, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(&a.qseq[slot], t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&a.subq[shard].count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&S->q_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     // The caller has already counted itself in S->active.
