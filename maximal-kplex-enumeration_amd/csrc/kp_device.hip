@@ -658,8 +658,9 @@ struct Worker {
     }
     __device__ __forceinline__ void set_error(int code)
     {
-        if (lane0()) atomicCAS(&S->error, 0, code);
         err = code;
+        if (lane0()) atomicCAS(&S->error, 0, code);
+        asm volatile("");
     }
 
     // ---- frame slots (only lane 0 touches them; values broadcast with readfirstlane) ----
@@ -928,23 +929,33 @@ struct Worker {
         const uint64_t *C4 = C2 + size_t(n) * W;
         LA = C4 + size_t(n) * W;
         const uint32_t *og = reinterpret_cast<const uint32_t *>(LA + size_t(l) * W);
+        // Lanes past the end load the last row and zero it: no lane-dependent
+        // branch, so the caller's join after a root switch stays uniform.
 #pragma unroll
         for (int j = 0; j < W; j++) {
             const uint32_t v = 64 * j + ln;
             const bool in = v < n;
+            const size_t vv = size_t(min(v, n - 1));
 #pragma unroll
             for (int w = 0; w < W; w++) {
-                adjR[j][w] = in ? A[size_t(v) * W + w] : 0ull;
-                c2R[j][w] = in ? C2[size_t(v) * W + w] : 0ull;
-                c4R[j][w] = in ? C4[size_t(v) * W + w] : 0ull;
+                const uint64_t x = A[vv * W + w], y = C2[vv * W + w], z = C4[vv * W + w];
+                adjR[j][w] = in ? x : 0ull;
+                c2R[j][w] = in ? y : 0ull;
+                c4R[j][w] = in ? z : 0ull;
             }
-            origR[j] = in ? og[v] : 0u;
+            const uint32_t o = og[vv];
+            origR[j] = in ? o : 0u;
         }
 #pragma unroll
         for (int t = 0; t < LW; t++) {
             const uint32_t u = 64 * t + ln;
+            const size_t uu = size_t(min(u, l - 1));
 #pragma unroll
-            for (int w = 0; w < W; w++) lftR[t][w] = u < l ? LA[size_t(u) * W + w] : 0ull;
+            for (int w = 0; w < W; w++) {
+                uint64_t x = 0;
+                if (l) x = LA[uu * W + w];
+                lftR[t][w] = u < l ? x : 0ull;
+            }
         }
         cur = ridx;
     }
@@ -1084,6 +1095,7 @@ struct Worker {
                 if (has(Sx, j)) a.list[o + __popcll(Sx[j] & lt_mask())] = origR[j];
                 o += __popcll(Sx[j]);
             }
+            asm volatile("");   // the per-lane stores join here, before any other path merges
         }
     }
 
@@ -1360,17 +1372,17 @@ struct Worker {
     __device__ void run()
     {
         bool active_self = false, hungry_self = false, have = false, paused = false;
-        uint32_t since_donate = 0, backoff = 1;
+        uint32_t to_donate = a.donate_period, backoff = 1;   // frames left until the next donation check
         uint32_t to_check = rfl(cold->check_every);   // frames left until the next check
         const uint32_t gw = rfl(blockIdx.x * 4u + (threadIdx.x >> 6));
-        Frame<W> f, nf;
+        Frame<W> f;
         for (;;) {
             launder();
             launder(f);
             have = uflag(have);
             active_self = uflag(active_self);
             hungry_self = uflag(hungry_self);
-            since_donate = rfl(since_donate);
+            to_donate = rfl(to_donate);
             to_check = rfl(to_check);
             backoff = rfl(backoff);
             if (!have) {
@@ -1483,12 +1495,30 @@ struct Worker {
                 }
                 if (!have) pop_top(f);
             }
-            if (f.root != cur) load_root(f.root);
-            have = (f.type == FT_KSEARCH) ? ksearch(f, nf) : branch(f, nf);
+            // A burst of frames from the wave's own stack, up to the next
+            // donation or check point.  The burst loop does no cross-wave work,
+            // so its control flow stays uniform and its state scalar.
+            const uint32_t burst = min(to_donate, to_check);
+            uint32_t nfr = 0;
+            for (;;) {
+                if (f.root != cur) load_root(f.root);
+                Frame<W> nf;
+                have = (f.type == FT_KSEARCH) ? ksearch(f, nf) : branch(f, nf);
+                nfr++;
+                if (err) break;
+                if (have) f = nf;
+                if (nfr == burst) break;
+                if (!have) {
+                    if (depth() == 0) break;
+                    pop_top(f);
+                    have = true;
+                }
+            }
             if (err) break;
-            if (have) f = nf;
-            if (++since_donate >= a.donate_period) {
-                since_donate = 0;
+            to_donate -= nfr;
+            to_check -= nfr;
+            if (to_donate == 0) {
+                to_donate = a.donate_period;
                 const uint32_t avail = depth() + (have ? 1u : 0u);
                 if (avail >= 2 && depth() >= 1) {
                     uint32_t want = 0;
@@ -1511,7 +1541,7 @@ struct Worker {
                     if (err) break;
                 }
             }
-            if (--to_check == 0) {
+            if (to_check == 0) {
                 const uint32_t ce = rfl(cold->check_every);
                 to_check = ce;
                 cold->frames = cold->frames + ce;
