@@ -559,8 +559,10 @@ struct Cold {
     uint32_t cjob, check_every;
 };
 
-// LW: left-vertex rows held in registers (left vertices 64t+lane, t < LW)
-template <int W, int LW>
+// LW: left-vertex rows held in registers (left vertices 64t+lane, t < LW);
+// LISTS: the kernel writes plex lists (kept out of the list-free kernel, whose
+// frame loop then holds no single-lane branch)
+template <int W, int LW, bool LISTS>
 struct Worker {
     static constexpr int FW = FrameLayout<W>::FW;
     const SearchArgs &a;
@@ -656,51 +658,83 @@ struct Worker {
     {
         return a.limit_ticks && wall_clock64() - rfl64(cold->t_start) > a.limit_ticks;
     }
+    // every lane makes the same compare-and-swap: the first one wins and the
+    // rest find the same code, so no lane-dependent branch is needed
     __device__ __forceinline__ void set_error(int code)
     {
         err = code;
-        if (lane0()) atomicCAS(&S->error, 0, code);
-        asm volatile("");
+        atomicCAS(&S->error, 0, code);
     }
 
-    // ---- frame slots (only lane 0 touches them; values broadcast with readfirstlane) ----
+    // ---- frame slots ----
+    // Slots of up to 64 words (W <= 4) are moved one word per lane with no
+    // branch: lane j holds word j, and lanes past the last word repeat the
+    // pad word (slot word FW-1, never read) with the same value, so every
+    // lane's store is harmless.  Wider slots go through lane 0.  Loaded words
+    // are broadcast with readlane / readfirstlane.
+    static constexpr bool kLaneSlots = FW <= 64 && W <= 4;
     __device__ __forceinline__ void store_slot(uint64_t *dst, uint64_t meta, const uint64_t *P, const uint64_t *C,
                                                const uint64_t *X, const uint64_t *D, bool withD)
     {
-        if (lane0()) {
-            dst[0] = meta;
+        if constexpr (kLaneSlots) {
+            uint64_t v = meta;
 #pragma unroll
             for (int w = 0; w < W; w++) {
-                dst[1 + w] = P[w];
-                dst[1 + W + w] = C[w];
-                dst[1 + 2 * W + w] = X[w];
-                if (withD) dst[1 + 3 * W + w] = D[w];
+                v = inmask(1ull << (1 + w)) ? P[w] : v;
+                v = inmask(1ull << (1 + W + w)) ? C[w] : v;
+                v = inmask(1ull << (1 + 2 * W + w)) ? X[w] : v;
+                if (withD) v = inmask(1ull << (1 + 3 * W + w)) ? D[w] : v;
             }
+            dst[min(ln, uint32_t(FW - 1))] = v;
+        } else {
+            if (lane0()) {
+                dst[0] = meta;
+#pragma unroll
+                for (int w = 0; w < W; w++) {
+                    dst[1 + w] = P[w];
+                    dst[1 + W + w] = C[w];
+                    dst[1 + 2 * W + w] = X[w];
+                    if (withD) dst[1 + 3 * W + w] = D[w];
+                }
+            }
+            asm volatile("");
         }
     }
     __device__ __forceinline__ void load_slot(const uint64_t *src, Frame<W> &f)
     {
-        uint64_t meta = 0;
-        uint64_t t[4 * W];
-        if (lane0()) {
-            meta = src[0];
+        if constexpr (kLaneSlots) {
+            const uint64_t x = src[min(ln, uint32_t(FW - 1))];
+            const uint64_t meta = rdl64<W>(x, 0);
+            f.root = uint32_t(meta);
+            f.type = uint32_t(meta >> 32) & 0xff;
+            f.r = uint32_t(meta >> 40) & 0xff;
 #pragma unroll
-            for (int i = 0; i < 3 * W; i++) t[i] = src[1 + i];
-        }
-        meta = rfl64(meta);
-        f.root = uint32_t(meta);
-        f.type = uint32_t(meta >> 32) & 0xff;
-        f.r = uint32_t(meta >> 40) & 0xff;
-        if (f.type == FT_KSEARCH && lane0()) {
+            for (int w = 0; w < W; w++) {
+                f.P[w] = rdl64<W>(x, 1 + w);
+                f.C[w] = rdl64<W>(x, 1 + W + w);
+                f.X[w] = rdl64<W>(x, 1 + 2 * W + w);
+                f.D[w] = f.type == FT_KSEARCH ? rdl64<W>(x, 1 + 3 * W + w) : 0ull;
+            }
+        } else {
+            uint64_t meta = 0;
+            uint64_t t[4 * W];
+            if (lane0()) {
+                meta = src[0];
 #pragma unroll
-            for (int i = 0; i < W; i++) t[3 * W + i] = src[1 + 3 * W + i];
-        }
+                for (int i = 0; i < 4 * W; i++) t[i] = src[1 + i];
+            }
+            asm volatile("");
+            meta = rfl64(meta);
+            f.root = uint32_t(meta);
+            f.type = uint32_t(meta >> 32) & 0xff;
+            f.r = uint32_t(meta >> 40) & 0xff;
 #pragma unroll
-        for (int w = 0; w < W; w++) {
-            f.P[w] = rfl64(t[w]);
-            f.C[w] = rfl64(t[W + w]);
-            f.X[w] = rfl64(t[2 * W + w]);
-            f.D[w] = f.type == FT_KSEARCH ? rfl64(t[3 * W + w]) : 0ull;
+            for (int w = 0; w < W; w++) {
+                f.P[w] = rfl64(t[w]);
+                f.C[w] = rfl64(t[W + w]);
+                f.X[w] = rfl64(t[2 * W + w]);
+                f.D[w] = f.type == FT_KSEARCH ? rfl64(t[3 * W + w]) : 0ull;
+            }
         }
     }
     static __device__ __forceinline__ uint64_t make_meta(uint32_t root, uint32_t type, uint32_t r)
@@ -709,9 +743,15 @@ struct Worker {
     }
     __device__ __forceinline__ void copy_slot(uint64_t *dst, const uint64_t *src)
     {
-        if (lane0()) {
+        if constexpr (kLaneSlots) {
+            const uint32_t i = min(ln, uint32_t(FW - 1));
+            dst[i] = src[i];
+        } else {
+            if (lane0()) {
 #pragma unroll
-            for (int i = 0; i < FW; i++) dst[i] = src[i];
+                for (int i = 0; i < FW; i++) dst[i] = src[i];
+            }
+            asm volatile("");
         }
     }
     __device__ __forceinline__ uint64_t *ring_slot(uint32_t i) const
@@ -908,6 +948,9 @@ struct Worker {
     }
 
     // ---- root cache: this lane's rows of the current root ----
+    // MayFlush: the root may belong to another job than the accumulators
+    // (then they are flushed first); callers that pass false have checked
+    template <bool MayFlush>
     __device__ void load_root(uint32_t ridx)
     {
         const RootDesc *d = a.desc + ridx;
@@ -919,9 +962,11 @@ struct Worker {
         k = int(kq & 0xffffu);
         q = int(kq >> 16);
         close_alg();
-        if (job != rfl(cold->cjob)) {
-            flush();
-            cold->cjob = job;
+        if constexpr (MayFlush) {
+            if (job != rfl(cold->cjob)) {
+                flush();
+                cold->cjob = job;
+            }
         }
         cold->bpt = 2ull * (5ull * n + 16ull);
         const uint64_t *A = reinterpret_cast<const uint64_t *>(rfl64(d->addr));
@@ -1077,7 +1122,7 @@ struct Worker {
         cnt++;
         ha += mix64(sa + uint64_t(s) * kp::HASH_LA);
         hb += mix64(sb + uint64_t(s) * kp::HASH_LB);
-        if (a.list) {
+        if constexpr (LISTS) {
             uint64_t base = 0;
             if (lane0()) base = atomicAdd((unsigned long long *)a.list_used, (unsigned long long)(s + 2));
             base = rfl64(base);
@@ -1499,9 +1544,16 @@ struct Worker {
             // donation or check point.  The burst loop does no cross-wave work,
             // so its control flow stays uniform and its state scalar.
             const uint32_t burst = min(to_donate, to_check);
+            if (f.root != cur) load_root<true>(f.root);
+            have = true;   // f holds the next frame
             uint32_t nfr = 0;
             for (;;) {
-                if (f.root != cur) load_root(f.root);
+                if (f.root != cur) {
+                    // a root of another job: its accumulators are flushed
+                    // outside the burst (the frame stays pending in f)
+                    if (rfl(a.desc[f.root].job) != rfl(cold->cjob)) break;
+                    load_root<false>(f.root);
+                }
                 Frame<W> nf;
                 have = (f.type == FT_KSEARCH) ? ksearch(f, nf) : branch(f, nf);
                 nfr++;
@@ -1601,7 +1653,7 @@ struct Worker {
 // The one-word class runs at 4 waves per SIMD (<= 128 VGPRs, no scratch).
 // The two-word class stays at its natural 2: capped at 3 it fits 168 VGPRs
 // without scratch but measured slower (soc-epinions 470 ms against 355 ms).
-template <int W, int LW>
+template <int W, int LW, bool LISTS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1 ? 4 : 1))) void search_kernel(SearchArgs a)
 {
     extern __shared__ uint64_t lds[];
@@ -1615,7 +1667,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1 ? 4 
     if (gw >= a.nwaves) return;
     uint64_t *ring = lds + size_t(wv) * a.ring_cap * FW;
     uint64_t *back = a.gstack ? a.gstack + size_t(gw) * a.back_cap * FW : nullptr;
-    Worker<W, LW> wk(a, ring, back, &s_cold[wv]);
+    Worker<W, LW, LISTS> wk(a, ring, back, &s_cold[wv]);
     wk.run();
 }
 
@@ -2093,15 +2145,22 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     ring = std::min(std::min(ring, ring_max), need);
     const uint32_t back = need > ring ? need - ring + 1 : 0;
     const size_t dyn = size_t(ring) * FW * 8 * 4;
-    auto kern = search_kernel<W, LW>;
+    auto kern = o.want_list ? search_kernel<W, LW, true> : search_kernel<W, LW, false>;
     int nb = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, dyn));
     if (nb < 1) nb = 1;
     // Untuned, when the epoch's largest class is a wide one (two or more
     // words, so at most two blocks per CU and VGPRs to spare), each other
     // class gets one block per CU and runs beside it instead of after it.
+    // Untuned, a wide class runs at most two blocks per CU and the one-word
+    // class at most four: roots are few and heavy-tailed, and more waves
+    // mostly poll and pull donations (measured: soc-epinions with the
+    // two-word class primary, 198 ms at four blocks and 166 ms at two; a
+    // 4,357-vertex graph of the many-graph workload, 36 ms at four one-word
+    // blocks and 195 ms at seven).
     if (T.blocks_per_cu > 0) nb = std::min(nb, T.blocks_per_cu);
     else if (secondary) nb = 1;
+    else nb = std::min(nb, W >= 2 ? 2 : 4);
     const uint32_t nwg = uint32_t(nb) * uint32_t(E.cus);
     SearchArgs sa{};
     sa.nwaves = nwg * 4;
