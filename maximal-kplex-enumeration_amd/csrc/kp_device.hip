@@ -2152,15 +2152,17 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     // Untuned, when the epoch's largest class is a wide one (two or more
     // words, so at most two blocks per CU and VGPRs to spare), each other
     // class gets one block per CU and runs beside it instead of after it.
-    // Untuned, a wide class runs at most two blocks per CU and the one-word
-    // class at most four: roots are few and heavy-tailed, and more waves
-    // mostly poll and pull donations (measured: soc-epinions with the
-    // two-word class primary, 198 ms at four blocks and 166 ms at two; a
-    // 4,357-vertex graph of the many-graph workload, 36 ms at four one-word
-    // blocks and 195 ms at seven).
+    // Untuned, a class with fewer pending roots and frames than the waves of
+    // a full grid runs at most four blocks per CU (one-word class) or two
+    // (wide classes): its few heavy-tailed roots leave most waves polling and
+    // pulling donations (measured: soc-epinions with the two-word class
+    // primary, 198 ms at four blocks and 166 ms at two; a 4,357-vertex graph
+    // of the many-graph workload, 36 ms at four one-word blocks and 195 ms at
+    // seven).  Batches and served epochs with many roots keep the full grid.
+    const uint64_t pending = uint64_t(C.htodo.size() - C.todo_done) + C.nseed + C.spill_n;
     if (T.blocks_per_cu > 0) nb = std::min(nb, T.blocks_per_cu);
     else if (secondary) nb = 1;
-    else nb = std::min(nb, W >= 2 ? 2 : 4);
+    else if (pending < uint64_t(nb) * uint64_t(E.cus) * 4u) nb = std::min(nb, W >= 2 ? 2 : 4);
     const uint32_t nwg = uint32_t(nb) * uint32_t(E.cus);
     SearchArgs sa{};
     sa.nwaves = nwg * 4;
