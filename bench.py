@@ -248,6 +248,40 @@ def spill_line(time_limit, pause_every=200_000_000, queue_capacity=4096, thres=0
             "plexes_per_s_spill": sp["count"] / (sp["search_ms"] / 1e3), "identical": True}
 
 
+def small_line(reps=7):
+    """The small planted workload (G(2000, 0.01) + 20 planted groups of 40,
+    k=3 q=20) end to end, in the reference's timing scope (preprocessing +
+    enumeration): the plain entry point on the GPU against the CPU oracle on
+    all host cores, medians of `reps` runs after a warm-up, same results."""
+    import statistics
+    import kplex
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import kpo
+    g, k, q, _, tr = kplex.workload_er2000()
+    kplex.enumerate_plexes(g, k, q)
+    gpu_s, r = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = kplex.enumerate_plexes(g, k, q)
+        gpu_s.append(time.perf_counter() - t0)
+    deg, nbr = g.csr()
+    og = kpo.OGraph.from_csr(deg, nbr)
+    threads = os.cpu_count() or 1
+    kpo.enumerate_plexes(og, k, q, tr, threads=threads)
+    cpu_s, c = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        c = kpo.enumerate_plexes(og, k, q, tr, threads=threads)
+        cpu_s.append(time.perf_counter() - t0)
+    assert (r["count"], r["hash"], r["tasks"]) == (c["count"], c["hash"], c["tasks"]), "small workload differs"
+    g_med, c_med = statistics.median(gpu_s), statistics.median(cpu_s)
+    log(f"[bench] small: GPU {g_med * 1e3:.2f} ms, CPU oracle ({threads} threads) {c_med * 1e3:.2f} ms")
+    return {"workload": WORKLOADS["er2000"][0], "count": r["count"], "tasks": r["tasks"],
+            "gpu_wall_ms": g_med * 1e3, "gpu_preprocess_ms": r["preprocess_ms"],
+            "gpu_device_ms": r["decompose_ms"] + r["search_ms"], "cpu_wall_ms": c_med * 1e3, "cpu_threads": threads,
+            "gpu_vs_cpu": c_med / g_med, "identical": True}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +298,7 @@ def main():
     ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--no-autotune", action="store_true", help="time with the built-in launch settings")
     ap.add_argument("--no-spill-line", action="store_true", help="skip the pause/spill line on the 320k workload")
+    ap.add_argument("--no-small-line", action="store_true", help="skip the GPU-vs-CPU line on the small planted workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -343,6 +378,8 @@ def main():
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline (target {args.cpu_target_s} s)")
         out["cpu_baseline"] = cpu_baseline(graph, k, q, truss, args.cpu_target_s, window)
+    if not args.no_small_line:
+        out["small"] = small_line()
     if not args.no_spill_line:
         out["spill"] = spill_line(args.time_limit)
     if args.batch_graphs > 0:
