@@ -1633,11 +1633,13 @@ struct Worker {
         const uint64_t frames = rfl64(cold->frames) + (rfl(cold->check_every) - to_check);
         uint32_t last = 0;
         if (lane0()) {
-            atomicAdd((unsigned long long *)&S->busy_ticks, (unsigned long long)busy_t);
-            atomicAdd((unsigned long long *)&S->idle_ticks, (unsigned long long)idle_t);
-            atomicAdd((unsigned long long *)&S->frames, (unsigned long long)frames);
-            atomicAdd((unsigned long long *)&S->spills, (unsigned long long)spills);
-            atomicAdd((unsigned long long *)&S->pops, (unsigned long long)pops);
+            // zero terms are skipped: most waves of a launch over little work
+            // never ran a frame, and every add here hits the same line
+            if (busy_t) atomicAdd((unsigned long long *)&S->busy_ticks, (unsigned long long)busy_t);
+            if (idle_t) atomicAdd((unsigned long long *)&S->idle_ticks, (unsigned long long)idle_t);
+            if (frames) atomicAdd((unsigned long long *)&S->frames, (unsigned long long)frames);
+            if (spills) atomicAdd((unsigned long long *)&S->spills, (unsigned long long)spills);
+            if (pops) atomicAdd((unsigned long long *)&S->pops, (unsigned long long)pops);
             last = __hip_atomic_fetch_add(&S->exited, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT) + 1 == a.nwaves;
         }
         // The last wave out moves frames still queued to the dump: every other
