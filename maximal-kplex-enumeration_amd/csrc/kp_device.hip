@@ -503,7 +503,6 @@ struct SearchArgs {
     const RootDesc *desc;      // class root table
     const uint32_t *todo;      // roots to start, in order (indices into desc)
     uint32_t ntodo;
-    uint32_t nstatic;          // wave i < nstatic starts with todo[i]; the scheduler starts counted
     const uint64_t *seed;      // frames to resume
     uint32_t nseed;
     uint64_t *dump;            // frames written on pause
@@ -1422,13 +1421,6 @@ struct Worker {
         uint32_t to_check = rfl(cold->check_every);   // frames left until the next check
         const uint32_t gw = rfl(blockIdx.x * 4u + (threadIdx.x >> 6));
         Frame<W> f;
-        // The first roots go one to a wave without a claim: the scheduler
-        // starts with next_root and active already counting them, so a full
-        // grid does not open with thousands of atomics on one line.
-        if (gw < a.nstatic) {
-            active_self = true;
-            push_root(a.todo[gw]);
-        }
         for (;;) {
             launder();
             launder(f);
@@ -2225,11 +2217,7 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     sa.limit_ticks = o.limit_ticks;
     sa.host_pause = o.host_pause ? E.dpause : nullptr;
     sa.pause_frames = o.pause_frames;
-    sa.nstatic = uint32_t(std::min<size_t>(ntodo, sa.nwaves));
-    C.hsched = Sched{};
-    C.hsched.next_root = sa.nstatic;
-    C.hsched.active = sa.nstatic;
-    HIPCHK(hipMemcpyAsync(sa.sched, &C.hsched, sizeof(Sched), hipMemcpyHostToDevice, C.stream));
+    HIPCHK(hipMemsetAsync(sa.sched, 0, sizeof(Sched), C.stream));
     HIPCHK(hipMemsetAsync(sa.subq, 0, sizeof(SubQ) * NSUBQ, C.stream));
     hipLaunchKernelGGL(iota_kernel, dim3(uint32_t((nslots + 255) / 256)), dim3(256), 0, C.stream, sa.qseq,
                        uint32_t(nslots), sa.qcap);

@@ -101,7 +101,6 @@ struct ClassState {
     bool launched = false;
     uint32_t nwaves = 0, ring = 0, back = 0, dump_cap = 0, launch_todo = 0;
     hipEvent_t done_ev = nullptr;
-    kpd::Sched hsched{};                // initial scheduler state of the last launch (copy source)
     bool has_work() const { return todo_done < htodo.size() || nseed > 0 || spill_n > 0; }
     ~ClassState();
 };
