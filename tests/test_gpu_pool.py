@@ -88,6 +88,21 @@ def test_batch_many_chung_lu_graphs():
     assert tot > 0
 
 
+def test_batch_with_more_roots_than_resident_waves():
+    # The first 48 graphs of the many-graph workload: thousands of roots over
+    # several size classes whose concurrent grids do not all fit on the chip
+    # at once.  A wave must count as busy only once it runs, or idle resident
+    # waves of one class wait on waves of another that need their slots.
+    specs = kplex.batch_graph_specs()[:48]
+    graphs = [kplex.graph_from_spec(s) for s in specs]
+    say(f"batch of the first {len(graphs)} many-graph members")
+    rs = kplex.enumerate_batch(graphs, [dict(k=s["k"], q=s["q"], time_limit_s=LIMIT) for s in specs])
+    assert sum(r["count"] for r in rs) == 1584447134
+    for i in (0, 11, 47):
+        say(f"  plain member {i}")
+        same(rs[i], kplex.enumerate_plexes(graphs[i], specs[i]["k"], specs[i]["q"], time_limit_s=LIMIT))
+
+
 def mid_graph():
     """12 planted 40-vertex groups with 8 removed edges per vertex: k=3, q=18
     gives 6,524 maximal 3-plexes over 2.5 million Branch tasks."""
