@@ -1735,8 +1735,9 @@ namespace {
 
 size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
 
-// the reference's task-queue capacity MAX_CAP (queue A), the
-// default for kp_params.queue_capacity
+// the reference's task-queue capacity MAX_CAP = 2048 * 2048 (inc/common.h:11,
+// the overflow queue of inc/host_funcs.h:982), the default for
+// kp_params.queue_capacity
 constexpr uint64_t kDefaultQueueCap = 4194304;
 
 } // namespace
