@@ -15,6 +15,13 @@ import statistics
 import sys
 import time
 
+# The library runs the size classes of an epoch on separate streams (nine per
+# engine); HIP maps streams round-robin onto GPU_MAX_HW_QUEUES hardware
+# queues, and with 4 two classes can share one and run back to back
+# (soc-epinions: 210 ms at 4 queues, 165 ms at 16).  HIP reads the variable
+# once, when its runtime starts, so it is set before the library loads.
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "maximal-kplex-enumeration_amd"))
 
