@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -158,3 +159,20 @@ def test_server_needs_a_device():
     with pytest.raises(kplex.KplexError) as e:
         kplex.Server()
     assert e.value.code == kplex.KP_ENODEV
+
+
+def _hw_queues_after_load(env):
+    code = ("import ctypes, sys; sys.path.insert(0, sys.argv[1]); import kplex; kplex.load(); "
+            "libc = ctypes.CDLL(None); libc.getenv.restype = ctypes.c_char_p; "
+            "v = libc.getenv(b'GPU_MAX_HW_QUEUES'); print(v.decode() if v else '')")
+    out = subprocess.run([sys.executable, "-c", code, os.path.join(ROOT, "maximal-kplex-enumeration_amd")],
+                         env=env, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    return out.stdout.strip().splitlines()[-1] if out.stdout.strip() else ""
+
+
+def test_library_requests_hw_queues_only_when_unset():
+    env = {k: v for k, v in os.environ.items() if k != "GPU_MAX_HW_QUEUES"}
+    assert _hw_queues_after_load(env) == "16"
+    env["GPU_MAX_HW_QUEUES"] = "4"
+    assert _hw_queues_after_load(env) == "4"
