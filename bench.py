@@ -289,13 +289,34 @@ def small_line(reps=7):
             "gpu_vs_cpu": c_med / g_med, "identical": True}
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of `nbytes` (hipMemcpy: a copy kernel that reads
+    and writes exactly nbytes).  Run under rocprofv3 --pmc FETCH_SIZE or
+    WRITE_SIZE, it gives what those counters report per HBM byte, the factor
+    that turns a kernel's counter readings into HBM bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)):
+            raise SystemExit("hipMalloc failed")
+    if hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) or hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3):
+        raise SystemExit("hipMemset/hipMemcpy failed")
+    if hip.hipDeviceSynchronize():
+        raise SystemExit("hipDeviceSynchronize failed")
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copied {nbytes} bytes device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cl75k", choices=sorted(WORKLOADS))
-    ap.add_argument("--time-limit", type=float, default=300.0)
+    # below the 180 s a silent run may last: a stuck launch ends as KP_ETIMEDOUT
+    ap.add_argument("--time-limit", type=float, default=150.0)
     ap.add_argument("--cpu-target-s", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=48,
@@ -306,7 +327,12 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="time with the built-in launch settings")
     ap.add_argument("--no-spill-line", action="store_true", help="skip the pause/spill line on the 320k workload")
     ap.add_argument("--no-small-line", action="store_true", help="skip the GPU-vs-CPU line on the small planted workload")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device (counter calibration for the traffic figure) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(roots would shard across GPUs with one final 32-byte reduction)", file=sys.stderr)

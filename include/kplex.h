@@ -106,6 +106,11 @@ typedef struct kp_result {
     uint64_t nlisted;      /* KP_FLAG_LIST: plex i is list_ids[list_off[i] .. list_off[i+1]) */
     uint64_t *list_off;
     uint32_t *list_ids;
+    uint32_t pn_first;     /* vertices after the first (q-k)-core; with truss on the reference
+                              prints it and then pn after the second core (inc/host_funcs.h:115,
+                              called at :749 and :823) */
+    int32_t  status;       /* kp_server_poll: KP_OK, or the KP_E* code that failed this query
+                              (its counts are then zero) */
 } kp_result;
 /*
  * Hash: for each maximal k-plex S of original vertex ids
@@ -197,7 +202,11 @@ int  kp_server_submit(kp_server *s, uint32_t graph_id, const kp_params *p, uint6
  * has ended, admits waiting queries, starts the next epoch, requests a pause
  * when queries wait.  Returns up to cap completed queries; in each result
  * wall_ms is submit -> completion, search_ms admission -> completion and
- * decompose_ms the admission (decompose + build) time of its group. */
+ * decompose_ms the admission (decompose + build) time of its group.  A query
+ * that cannot run (preprocessing failed, a block larger than KP_MAX_BLOCK, a
+ * subgraph arena larger than the whole budget, an error reported by a search
+ * kernel) comes back with status set and zero counts; the call itself fails
+ * only when the HIP runtime does. */
 int  kp_server_poll(kp_server *s, uint64_t *tickets, kp_result *results, size_t cap, size_t *ndone);
 /* Number of queries submitted and not yet returned by kp_server_poll. */
 size_t kp_server_outstanding(const kp_server *s);

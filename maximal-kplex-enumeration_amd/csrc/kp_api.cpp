@@ -11,14 +11,6 @@ using namespace kp;
 
 namespace {
 
-// The size classes of an epoch run on their own streams (nine streams per
-// engine).  HIP maps streams round-robin onto GPU_MAX_HW_QUEUES hardware
-// queues, 4 by default, so two classes can land on one queue and run one
-// after the other.  Unless the process chose a value, ask for 16 when the
-// library is loaded: the HIP runtime reads the variable once, when it starts
-// (a process that started HIP before loading the library keeps its setting).
-__attribute__((constructor)) void request_hw_queues() { setenv("GPU_MAX_HW_QUEUES", "16", 0); }
-
 int check_params(const kp_graph *g, const kp_params *p)
 {
     if (!g || !p) return KP_EINVAL;
@@ -105,12 +97,9 @@ extern "C" int kp_enumerate(const kp_graph *g, const kp_params *p, kp_result *r)
 {
     if (!r) return KP_EINVAL;
     std::memset(r, 0, sizeof(*r));
-    kp_prepared *h = nullptr;
-    int rc = kp_prepare(g, p, &h);
+    int rc = check_params(g, p);
     if (rc) return rc;
-    rc = dev_run(h, r);
-    dev_free(h);
-    return rc;
+    return dev_enumerate(g, p, r);
 }
 
 extern "C" int kp_enumerate_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r)

@@ -34,6 +34,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -523,6 +525,7 @@ struct SearchArgs {
     uint64_t limit_ticks;      // 0 = no limit; else wall-clock ticks per launch
     const uint32_t *host_pause;   // mapped host word (nonzero = pause) or null
     uint64_t pause_frames;     // 0 = none; else pause once this many frames were processed
+    unsigned long long *t0;    // shared start tick of the epoch (0 until the first wave of any class starts)
 };
 
 enum : uint32_t { FT_BRANCH = 0, FT_KSEARCH = 1 };
@@ -592,7 +595,18 @@ struct Worker {
         k = 0;
         q = 0;
         const uint64_t now = wall_clock64();
-        cold->t_start = now;
+        // The time limit runs from the first wave of the epoch, over every
+        // class: a class whose grid waits for CUs held by another class does
+        // not restart the clock when it starts.
+        uint64_t start = now;
+        if (a.t0) {
+            if (lane0()) {
+                const unsigned long long old = atomicCAS(a.t0, 0ull, (unsigned long long)now);
+                start = old ? old : now;
+            }
+            start = rfl64(start);
+        }
+        cold->t_start = start;
         cold->t_mark = now;
         cold->busy_t = cold->idle_t = 0;
         cold->alg = cold->task0 = cold->bpt = cold->frames = cold->pops = 0;
@@ -707,7 +721,7 @@ struct Worker {
             const uint64_t meta = rdl64<W>(x, 0);
             f.root = uint32_t(meta);
             f.type = uint32_t(meta >> 32) & 0xff;
-            f.r = uint32_t(meta >> 40) & 0xff;
+            f.r = uint32_t(meta >> 40) & 0xffff;
 #pragma unroll
             for (int w = 0; w < W; w++) {
                 f.P[w] = rdl64<W>(x, 1 + w);
@@ -727,7 +741,7 @@ struct Worker {
             meta = rfl64(meta);
             f.root = uint32_t(meta);
             f.type = uint32_t(meta >> 32) & 0xff;
-            f.r = uint32_t(meta >> 40) & 0xff;
+            f.r = uint32_t(meta >> 40) & 0xffff;
 #pragma unroll
             for (int w = 0; w < W; w++) {
                 f.P[w] = rfl64(t[w]);
@@ -1673,6 +1687,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1 ? 4 
     wk.run();
 }
 
+// fills n words with v (a kernel rather than hipMemsetAsync, so that the
+// launch sequence stays capturable as a graph)
+__global__ void fill_u64_kernel(uint64_t *p, uint32_t n, uint64_t v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // slot j of every shard starts with sequence number j (free for ticket j)
 __global__ void iota_kernel(uint64_t *seq, uint32_t n, uint32_t per_shard)
 {
@@ -1760,7 +1782,7 @@ int upload_graphs(const std::vector<const Prep *> &preps, DevBuf &buf, std::vect
     if (rc) return rc;
     std::vector<uint8_t> host(total);
     views.assign(preps.size(), GraphView{});
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for schedule(dynamic, 1) if (preps.size() > 1)
     for (size_t i = 0; i < preps.size(); i++) {
         const Prep &P = *preps[i];
         const kp_graph &G = P.g;
@@ -1803,8 +1825,45 @@ ClassState::~ClassState()
 {
     if (spill) (void)hipHostFree(spill);
     if (done_ev) (void)hipEventDestroy(done_ev);
-    if (stream) (void)hipStreamDestroy(stream);
 }
+
+namespace {
+
+// The streams of one device, created once per process and shared by every
+// engine on it.  HIP maps streams onto its hardware queues (GPU_MAX_HW_QUEUES,
+// 4 by default) round-robin in creation order, so the streams of the four
+// narrowest size classes, which carry nearly all the work, are created first
+// and get a queue each; classes on one queue would run one after the other.
+// Engines on one device share these streams: their launches are ordered on
+// them, never lost.
+struct DeviceStreams {
+    bool ready = false;
+    int cus = 0;
+    hipStream_t cls[kNumClass] = {};
+    hipStream_t main = nullptr, side = nullptr;
+};
+std::mutex g_streams_mu;
+DeviceStreams g_streams[64];
+
+int device_streams(int device, DeviceStreams **out)
+{
+    if (device < 0 || device >= 64) return KP_ENODEV;
+    std::lock_guard<std::mutex> lk(g_streams_mu);
+    DeviceStreams &d = g_streams[device];
+    if (!d.ready) {
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        d.cus = prop.multiProcessorCount;
+        for (int c = 0; c < kNumClass; c++) HIPCHK(hipStreamCreateWithFlags(&d.cls[c], hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&d.main, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&d.side, hipStreamNonBlocking));
+        d.ready = true;
+    }
+    *out = &d;
+    return KP_OK;
+}
+
+} // namespace
 
 int Engine::init()
 {
@@ -1814,14 +1873,15 @@ int Engine::init()
         return KP_ENODEV;
     }
     HIPCHK(hipGetDevice(&device));
-    hipDeviceProp_t prop;
-    HIPCHK(hipGetDeviceProperties(&prop, device));
-    cus = prop.multiProcessorCount;
-    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-    for (auto &c : cls) {
-        HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-        HIPCHK(hipEventCreateWithFlags(&c.done_ev, hipEventDisableTiming));
+    DeviceStreams *ds = nullptr;
+    int rc = device_streams(device, &ds);
+    if (rc) return rc;
+    cus = ds->cus;
+    stream = ds->main;
+    side = ds->side;
+    for (int c = 0; c < kNumClass; c++) {
+        cls[c].stream = ds->cls[c];
+        if (!cls[c].done_ev) HIPCHK(hipEventCreateWithFlags(&cls[c].done_ev, hipEventDisableTiming));
     }
     return KP_OK;
 }
@@ -1830,8 +1890,6 @@ Engine::~Engine()
 {
     for (DevBuf *b : segs) delete b;
     if (hpause) (void)hipHostFree(hpause);
-    if (side) (void)hipStreamDestroy(side);
-    if (stream) (void)hipStreamDestroy(stream);
 }
 
 uint64_t Engine::device_bytes() const
@@ -1911,8 +1969,35 @@ void Engine::reset_tables()
         c.spill_n = 0;
         c.launched = false;
     }
+    segs_used = 0;
+}
+
+uint64_t *Engine::arena_alloc(uint64_t words)
+{
+    const size_t need = 8 * std::max<uint64_t>(words, 1);
+    size_t pick = segs.size();
+    for (size_t i = segs_used; i < segs.size(); i++)
+        if (segs[i]->bytes >= need && (pick == segs.size() || segs[i]->bytes < segs[pick]->bytes)) pick = i;
+    if (pick == segs.size()) {
+        auto *b = new (std::nothrow) DevBuf();
+        if (!b) return nullptr;
+        if (b->ensure(need)) {
+            delete b;
+            return nullptr;
+        }
+        segs.push_back(b);
+    }
+    std::swap(segs[pick], segs[segs_used]);
+    DevBuf *b = segs[segs_used++];
+    if (hipMemsetAsync(b->p, 0, need, stream) != hipSuccess) return nullptr;
+    return b->as<uint64_t>();
+}
+
+void Engine::release_arena()
+{
     for (DevBuf *b : segs) delete b;
     segs.clear();
+    segs_used = 0;
 }
 
 namespace {
@@ -2218,6 +2303,7 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     sa.limit_ticks = o.limit_ticks;
     sa.host_pause = o.host_pause ? E.dpause : nullptr;
     sa.pause_frames = o.pause_frames;
+    sa.t0 = E.t0.as<unsigned long long>();
     HIPCHK(hipMemsetAsync(sa.sched, 0, sizeof(Sched), C.stream));
     HIPCHK(hipMemsetAsync(sa.subq, 0, sizeof(SubQ) * NSUBQ, C.stream));
     hipLaunchKernelGGL(iota_kernel, dim3(uint32_t((nslots + 255) / 256)), dim3(256), 0, C.stream, sa.qseq,
@@ -2248,6 +2334,14 @@ int launch_class_w(Engine &E, int c, const SearchOpts &o, bool secondary = false
     case 16: return launch_class<16, 0>(E, c, o, secondary);
     }
     return KP_EINVAL;
+}
+
+// the epoch's shared start tick back to 0 (the first wave to start sets it)
+hipError_t zero_t0(Engine &E, hipStream_t st)
+{
+    if (E.t0.ensure(256) != KP_OK) return hipErrorOutOfMemory;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, st, E.t0.as<uint64_t>(), 1u, 0ull);
+    return hipGetLastError();
 }
 
 int grow_spill(ClassState &C, size_t need, size_t fbytes)
@@ -2318,6 +2412,7 @@ int carry_frames(Engine &E, ClassState &C, size_t fbytes, uint32_t D, uint32_t n
 int Engine::launch_epoch(const SearchOpts &o)
 {
     // the class streams start after everything queued on the main stream
+    HIPCHK(zero_t0(*this, stream));
     hipEvent_t ev;
     HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     HIPCHK(hipEventRecord(ev, stream));
@@ -2403,6 +2498,7 @@ int Engine::time_class(int c, const SearchOpts &o, float *ms)
         ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
     } eg{a, b};
     HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(zero_t0(*this, C.stream));
     HIPCHK(hipEventRecord(a, C.stream));
     int rc = launch_class_w(*this, c, o);
     if (rc) return rc;
@@ -2487,7 +2583,7 @@ int Engine::finish_epoch(const SearchOpts &o, EpochStats &es)
 // ----------------------------------------------------------------------
 struct kp_prepared {
     kp_params p{};
-    kp::Prep prep;
+    std::unique_ptr<kp::Prep> prep{new kp::Prep()};
     kp::DevBuf gbuf;
     kp::GraphView gv;
     kp::Engine E;
@@ -2524,11 +2620,11 @@ int dev_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out)
     int rc = h->E.init();
     if (rc) { delete h; return rc; }
     const double t0 = now_ms();
-    if ((rc = preprocess(g, p->k, p->q, p->truss, &h->prep))) { delete h; return rc; }
+    if ((rc = preprocess(g, p->k, p->q, p->truss, h->prep.get()))) { delete h; return rc; }
     h->preprocess_ms = now_ms() - t0;
     const double t1 = now_ms();
     std::vector<GraphView> views;
-    if ((rc = upload_graphs({&h->prep}, h->gbuf, views, h->E.stream))) { delete h; return rc; }
+    if ((rc = upload_graphs({h->prep.get()}, h->gbuf, views, h->E.stream))) { delete h; return rc; }
     h->gv = views[0];
     h->upload_ms = now_ms() - t1;
     *out = h;
@@ -2624,17 +2720,8 @@ int run_pool(Engine &E, const std::vector<JobIn> &jobs, const kp_params &P0, std
     } eg{e0, e1, e2};
     HIPCHK(hipEventRecord(e0, E.stream));
     std::vector<JobStats> st;
-    // plain and batched runs: one arena segment per staging chunk
-    auto alloc = [&E](size_t, uint64_t words) -> uint64_t * {
-        auto *b = new (std::nothrow) DevBuf();
-        if (!b) return nullptr;
-        if (b->ensure(8 * words) || hipMemsetAsync(b->p, 0, 8 * words, E.stream) != hipSuccess) {
-            delete b;
-            return nullptr;
-        }
-        E.segs.push_back(b);
-        return b->as<uint64_t>();
-    };
+    // plain and batched runs: one arena segment per staging chunk, reused across runs
+    auto alloc = [&E](size_t, uint64_t words) -> uint64_t * { return E.arena_alloc(words); };
     if ((rc = E.stage(jobs, st, alloc))) return rc;
     HIPCHK(hipEventRecord(e1, E.stream));
     int orc = KP_OK;
@@ -2707,10 +2794,80 @@ int dev_run(kp_prepared *h, kp_result *res)
     if ((rc = run_pool(h->E, {j}, h->p, rv, dms, sms))) return rc;
     res->preprocess_ms = h->preprocess_ms;
     res->upload_ms = h->upload_ms;
+    res->pn_first = h->prep->pn_first;
     res->decompose_ms = dms;
     res->search_ms = sms;
     res->wall_ms = res->preprocess_ms + dms + sms;
     return KP_OK;
+}
+
+namespace {
+
+// The engine behind kp_enumerate, kept per device between calls: its
+// streams, workspaces and arena segments are reused, so a call on a small
+// graph costs preprocessing, one upload and the kernels.  A call that finds
+// it busy (another host thread) runs on an engine of its own.
+struct EnumCache {
+    std::mutex mu;
+    kp_prepared *h = nullptr;
+};
+EnumCache g_enum_cache[64];
+constexpr uint64_t kEnumCacheKeepBytes = 1ull << 30;   // larger engines are freed after the call
+
+} // namespace
+
+int dev_enumerate(const kp_graph *g, const kp_params *p, kp_result *r)
+{
+    std::memset(r, 0, sizeof(*r));
+    int count = 0, dev = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        set_error("no HIP device visible: the HIP backend is required");
+        return KP_ENODEV;
+    }
+    HIPCHK(hipGetDevice(&dev));
+    std::unique_lock<std::mutex> lk;
+    if (dev < 64) lk = std::unique_lock<std::mutex>(g_enum_cache[dev].mu, std::try_to_lock);
+    if (!lk.owns_lock()) {
+        kp_prepared *h = nullptr;
+        int rc = dev_prepare(g, p, &h);
+        if (rc) return rc;
+        rc = dev_run(h, r);
+        dev_free(h);
+        return rc;
+    }
+    EnumCache &C = g_enum_cache[dev];
+    int rc;
+    if (!C.h) {
+        C.h = new (std::nothrow) kp_prepared();
+        if (!C.h) return KP_ENOMEM;
+        if ((rc = C.h->E.init())) {
+            delete C.h;
+            C.h = nullptr;
+            return rc;
+        }
+    }
+    kp_prepared *h = C.h;
+    h->p = *p;
+    h->prep.reset(new Prep());
+    const double t0 = now_ms();
+    rc = preprocess(g, p->k, p->q, p->truss, h->prep.get());
+    h->preprocess_ms = now_ms() - t0;
+    if (!rc) {
+        const double t1 = now_ms();
+        std::vector<GraphView> views;
+        rc = upload_graphs({h->prep.get()}, h->gbuf, views, h->E.stream);
+        if (!rc) {
+            h->gv = views[0];
+            h->upload_ms = now_ms() - t1;
+            rc = dev_run(h, r);
+        }
+    }
+    h->prep.reset(new Prep());
+    if (rc || dev_bytes(h) > kEnumCacheKeepBytes) {   // after an error the next call starts afresh
+        delete h;
+        C.h = nullptr;
+    }
+    return rc;
 }
 
 // Coordinate descent over the launch settings of each size class, timing the
@@ -2730,16 +2887,7 @@ int dev_autotune(kp_prepared *h, kp_tune *out)
     E.reset_tables();
     if ((rc = E.zero_acc(0, 1))) return rc;
     std::vector<JobStats> st;
-    auto alloc = [&E](size_t, uint64_t words) -> uint64_t * {
-        auto *b = new (std::nothrow) DevBuf();
-        if (!b) return nullptr;
-        if (b->ensure(8 * words) || hipMemsetAsync(b->p, 0, 8 * words, E.stream) != hipSuccess) {
-            delete b;
-            return nullptr;
-        }
-        E.segs.push_back(b);
-        return b->as<uint64_t>();
-    };
+    auto alloc = [&E](size_t, uint64_t words) -> uint64_t * { return E.arena_alloc(words); };
     if ((rc = E.stage({j}, st, alloc))) return rc;
     kp_params P = h->p;
     P.pause_every = 0;
@@ -2853,6 +3001,7 @@ int dev_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result 
     if ((rc = run_pool(E, jobs, p[0], rv, dms, sms))) return rc;
     for (size_t i = 0; i < n; i++) {
         r[i].preprocess_ms = pms[i];
+        r[i].pn_first = preps[i].pn_first;
         r[i].upload_ms = t2 - t1;
         r[i].decompose_ms = dms;
         r[i].search_ms = sms;

@@ -119,7 +119,9 @@ struct Engine {
     DevBuf blk, lft, info, slot_job, slot_pos, ws_cnt, ws_b1c, ws_a1, bdesc, wr_extra;
     size_t ws_cnt_zeroed = 0;          // bytes of ws_cnt known to be zero
     std::vector<DevBuf *> segs;        // arena segments owned by the engine (plain / batch)
+    size_t segs_used = 0;              // segs[0, segs_used) hold the current tables' subgraphs
     DevBuf list, list_used;
+    DevBuf t0;                         // shared start tick of the running epoch (time limit)
     ClassState cls[kNumClass];
     uint32_t *hpause = nullptr, *dpause = nullptr;   // pinned host pause word
     uint64_t total_spilled = 0, total_pauses = 0;
@@ -142,6 +144,10 @@ struct Engine {
     int finish_epoch(const SearchOpts &o, EpochStats &es);
     bool has_work() const;
     void reset_tables();               // forget every root table, start list and pending frame
+    // zeroed device memory for `words` 64-bit words of subgraph arena, from a
+    // segment kept across runs (reset_tables frees them all for reuse)
+    uint64_t *arena_alloc(uint64_t words);
+    void release_arena();              // gives the segments back to the device
     int read_acc(uint32_t first, uint32_t count, std::vector<uint64_t> &out5);   // 5 words per job
     // pending frames (device seeds and host spill) per job index; per_job must have room
     int count_pending(std::vector<uint64_t> &per_job);

@@ -33,6 +33,7 @@ struct Prep {
     kp_graph g{};
     std::vector<uint32_t> orig;
     std::vector<int32_t> dseq, dpos;
+    uint32_t pn_first = 0;   // vertices after the first core (before the truss)
     Prep() = default;
     Prep(const Prep &) = delete;
     Prep &operator=(const Prep &) = delete;

@@ -38,6 +38,9 @@ double now_ms()
 
 namespace {
 
+// adjacency entries below which preprocessing stays on the calling thread
+constexpr uint64_t kParallelEntries = 1u << 18;
+
 // The bd-core is unique, so the parallel cascade below gives the same vertex
 // set as the reference's per-thread queues with CAS decrements.
 int core_peel(const kp_graph *g, int bd, kp_graph *out, std::vector<uint32_t> &old_of_new)
@@ -51,9 +54,12 @@ int core_peel(const kp_graph *g, int bd, kp_graph *out, std::vector<uint32_t> &o
         alive[i] = int64_t(g->deg[i]) >= bd;
         if (!alive[i]) frontier.push_back(i);
     }
+    // Small inputs run on the calling thread: waking an OpenMP team costs
+    // more than peeling a few thousand vertices.
+    const bool par = g->m > kParallelEntries;
     while (!frontier.empty()) {
         next.clear();
-        #pragma omp parallel
+        #pragma omp parallel if (par && frontier.size() > 1024)
         {
             std::vector<uint32_t> mine;
             #pragma omp for schedule(dynamic, 256) nowait
@@ -85,7 +91,7 @@ int core_peel(const kp_graph *g, int bd, kp_graph *out, std::vector<uint32_t> &o
         }
     const uint32_t pn = uint32_t(old_of_new.size());
     std::vector<uint32_t> ndeg(pn);
-    #pragma omp parallel for schedule(dynamic, 256)
+    #pragma omp parallel for schedule(dynamic, 256) if (par)
     for (long i = 0; i < long(pn); i++) {
         uint32_t o = old_of_new[i], c = 0;
         for (uint64_t p = g->off[o]; p < g->off[o + 1]; p++) c += alive[g->nbr[p]];
@@ -100,7 +106,7 @@ int core_peel(const kp_graph *g, int bd, kp_graph *out, std::vector<uint32_t> &o
         out->deg[i] = ndeg[i];
         out->off[i + 1] = out->off[i] + ndeg[i];
     }
-    #pragma omp parallel for schedule(dynamic, 256)
+    #pragma omp parallel for schedule(dynamic, 256) if (par)
     for (long i = 0; i < long(pn); i++) {
         uint32_t o = old_of_new[i];
         uint64_t w = out->off[i];
@@ -129,7 +135,7 @@ int truss_prefilter(kp_graph *g, int q, int k)
     const uint64_t m = g->m;
     const int thr = q - 2 * k;
     std::vector<int32_t> tri(m);
-    #pragma omp parallel for schedule(dynamic, 64)
+    #pragma omp parallel for schedule(dynamic, 64) if (m > kParallelEntries)
     for (long i = 0; i < long(n); i++) {
         for (uint64_t p = g->off[i]; p < g->off[i + 1]; p++) {
             const uint32_t v = g->nbr[p];
@@ -240,6 +246,7 @@ int preprocess(const kp_graph *g, int k, int q, int truss, Prep *out)
     std::vector<uint32_t> o1;
     int rc = core_peel(g, bd, &out->g, o1);
     if (rc) return rc;
+    out->pn_first = out->g.n;
     if (truss) {
         rc = truss_prefilter(&out->g, q, k);
         if (rc) return rc;

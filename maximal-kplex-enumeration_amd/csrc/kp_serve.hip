@@ -193,6 +193,7 @@ int complete(kp_server *s, uint32_t slot, double now)
     r.max_l = L.st.max_l;
     r.pn = L.res->gv.pn;
     r.pm = L.res->gv.pm;
+    r.pn_first = L.res->prep.pn_first;
     r.preprocess_ms = L.prep_ms;
     r.decompose_ms = L.stage_ms;
     r.search_ms = now - L.t_admit;
@@ -206,7 +207,35 @@ int complete(kp_server *s, uint32_t slot, double now)
     return KP_OK;
 }
 
-// Finishes the ended epoch: carries frames, retires complete queries.
+// A query that cannot run leaves its slot with an error result (status = code,
+// counts zero).
+void fail_query(kp_server *s, uint32_t slot, int code, double now)
+{
+    Live &L = s->live[slot];
+    kp_result r;
+    std::memset(&r, 0, sizeof r);
+    r.status = code;
+    r.wall_ms = now - L.q.t_submit;
+    s->done.emplace_back(L.q.ticket, r);
+    free_regions(s, L);
+    L = Live{};
+    s->free_slots.push_back(slot);
+}
+
+// A query refused before admission (no slot) leaves with an error result.
+void fail_waiting(kp_server *s, const Query &q, int code)
+{
+    kp_result r;
+    std::memset(&r, 0, sizeof r);
+    r.status = code;
+    r.wall_ms = now_ms() - q.t_submit;
+    s->done.emplace_back(q.ticket, r);
+}
+
+// Finishes the ended epoch: carries frames, retires complete queries.  If a
+// search kernel reported an error, every admitted query fails: their waves
+// may already have added partial counts to the accumulators, and resuming
+// the epoch's roots would count them again.
 int finish(kp_server *s)
 {
     Engine &E = s->E;
@@ -215,7 +244,13 @@ int finish(kp_server *s)
     EpochStats es;
     int rc = E.finish_epoch(s->opts, es);
     s->epoch_running = false;
-    if (rc) return rc;
+    if (rc) {
+        const double now = now_ms();
+        for (uint32_t j = 0; j < s->live.size(); j++)
+            if (s->live[j].used) fail_query(s, j, rc, now);
+        E.reset_tables();
+        return rc == KP_EHIP ? rc : KP_OK;
+    }
     for (int c = 0; c < kNumClass; c++) {
         const ClassState &C = E.cls[c];
         for (size_t t = before[c]; t < C.todo_done; t++) {
@@ -235,6 +270,66 @@ int finish(kp_server *s)
     return KP_OK;
 }
 
+// Records a staged query's statistics at admission.
+void admitted(kp_server *s, uint32_t slot, const JobStats &st, double stage_ms, double now)
+{
+    Live &L = s->live[slot];
+    L.st = st;
+    L.roots_left = st.valid_roots;
+    L.stage_ms = stage_ms;
+    L.t_admit = now;
+    L.pauses0 = s->E.total_pauses;
+    L.spilled0 = s->E.total_spilled;
+}
+
+// Stages the queries of `part` one at a time after their group failed for a
+// reason other than arena memory.  A query that fails alone leaves with an
+// error result; one that does not fit the arena goes back to the queue with
+// every query after it.
+int admit_one_by_one(kp_server *s, const std::vector<JobIn> &part, double t0)
+{
+    Engine &E = s->E;
+    for (size_t i = 0; i < part.size(); i++) {
+        const uint32_t slot = part[i].id;
+        bool oom = false;
+        auto alloc = [&](size_t, uint64_t words) -> uint64_t * {
+            uint64_t *p = s->pool.alloc(8 * words);
+            if (!p) {
+                oom = true;
+                return nullptr;
+            }
+            if (hipMemsetAsync(p, 0, 8 * words, E.stream) != hipSuccess) return nullptr;
+            s->live[slot].regions.emplace_back(p, 8 * words);
+            return p;
+        };
+        std::vector<JobStats> st;
+        const int rc = E.stage({part[i]}, st, alloc);
+        const double now = now_ms();
+        if (rc == KP_EHIP) return rc;
+        if (rc && oom) {
+            // this query and the ones after it wait for arena memory
+            for (size_t j = part.size(); j-- > i;) {
+                Live &L = s->live[part[j].id];
+                free_regions(s, L);
+                s->pending.push_front(L.q);
+                s->free_slots.push_back(part[j].id);
+                L = Live{};
+            }
+            return KP_OK;
+        }
+        if (rc) {
+            fail_query(s, slot, rc, now);
+            continue;
+        }
+        admitted(s, slot, st[0], now - t0, now);
+        if (st[0].valid_roots == 0) {
+            int rc2 = complete(s, slot, now);
+            if (rc2) return rc2;
+        }
+    }
+    return KP_OK;
+}
+
 // Admits waiting queries while slots and arena memory allow.
 int admit(kp_server *s)
 {
@@ -248,9 +343,10 @@ int admit(kp_server *s)
         int rc = get_resident(s, q, &res, &pms);
         if (rc) {
             // the query cannot run (its preprocessing failed): it leaves the
-            // queue and the error is returned once
+            // queue with an error result
+            fail_waiting(s, q, rc);
             s->pending.pop_front();
-            return rc;
+            continue;
         }
         const uint32_t slot = s->free_slots.back();
         s->free_slots.pop_back();
@@ -273,7 +369,9 @@ int admit(kp_server *s)
         if ((rc = E.zero_acc(slot, 1))) return rc;
     const double t0 = now_ms();
     // Stage the group; if the arena cannot hold it, the queries that do not
-    // fit go back to the front of the queue, one by one from the end.
+    // fit go back to the front of the queue, one by one from the end.  Any
+    // other failure is isolated by staging the queries one at a time: a query
+    // that fails alone leaves with an error result, the others run.
     size_t take = jobs.size();
     std::vector<JobStats> st;
     for (;;) {
@@ -292,7 +390,9 @@ int admit(kp_server *s)
         rc = E.stage(part, st, alloc);
         if (!rc) break;
         for (size_t i = 0; i < take; i++) free_regions(s, s->live[part[i].id]);
-        if (!oom || take == 0) return rc;
+        if (rc == KP_EHIP) return rc;
+        if (!oom) return admit_one_by_one(s, part, t0);
+        if (take == 0) return rc;
         // put the last query of the group back
         Live &L = s->live[jobs[take - 1].id];
         s->pending.push_front(L.q);
@@ -305,23 +405,16 @@ int admit(kp_server *s)
             if (!any) {
                 set_error("query %llu needs more subgraph arena than the server budget of %llu bytes",
                           (unsigned long long)s->pending.front().ticket, (unsigned long long)s->pool.cap);
+                fail_waiting(s, s->pending.front(), KP_ENOMEM);
                 s->pending.pop_front();
-                return KP_ENOMEM;
+                return KP_OK;
             }
             return KP_OK;   // waits until running queries free the arena
         }
     }
     const double ms = now_ms() - t0;
     const double now = now_ms();
-    for (size_t i = 0; i < take; i++) {
-        Live &L = s->live[jobs[i].id];
-        L.st = st[i];
-        L.roots_left = st[i].valid_roots;
-        L.stage_ms = ms;
-        L.t_admit = now;
-        L.pauses0 = E.total_pauses;
-        L.spilled0 = E.total_spilled;
-    }
+    for (size_t i = 0; i < take; i++) admitted(s, jobs[i].id, st[i], ms, now);
     for (size_t i = 0; i < take; i++)
         if (s->live[jobs[i].id].roots_left == 0)
             if ((rc = complete(s, jobs[i].id, now))) return rc;

@@ -1,8 +1,8 @@
 // main.cpp -- command-line front end with the reference's interface
 // (main.cu:11-62):  PlexEnum <dataset> -k <k> -q <q> -t <thres> -tr <truss>
 // It prints the reference's lines (main.cu:30,54; inc/graph.h:47;
-// inc/host_funcs.h:115,1149,1156-1157), with a 64-bit plex count and one
-// extra "hash:" line.
+// inc/host_funcs.h:115,1149,1156-1158), with a 64-bit plex count and one
+// extra "hash:" line at the end.
 #include "../../include/kplex.h"
 
 #include <cstdio>
@@ -67,11 +67,15 @@ int main(int argc, char *argv[])
         kp_graph_free(&g);
         return 1;
     }
-    std::printf("\npn = %u\n", r.pn);
+    // one "pn =" line per core peel: after the first core, and with the
+    // truss on again after the second core (inc/host_funcs.h:115, :749, :823)
+    std::printf("\npn = %u\n", r.pn_first);
+    if (p.truss) std::printf("\npn = %u\n", r.pn);
     std::printf("Total Time Elapsed: %f ms\n", r.wall_ms);
     std::printf("Total Valid Blocks: %llu, Maximal k-Plexes: %llu\n", (unsigned long long)r.valid_roots,
                 (unsigned long long)r.count);
     std::printf("Total tasks generated: %llu\n", (unsigned long long)r.tasks);
+    std::printf("\nKernel Launch Successfully\n");
     std::printf("hash: %016llx%016llx\n", (unsigned long long)r.hash_b, (unsigned long long)r.hash_a);
     kp_result_free(&r);
     kp_graph_free(&g);

@@ -48,7 +48,8 @@ class KpResult(ctypes.Structure):
                 ("max_n", c_uint32), ("max_l", c_uint32), ("alg_bytes", c_uint64), ("adj_bytes", c_uint64), ("search_alg_bytes", c_uint64),
                 ("preprocess_ms", c_double), ("upload_ms", c_double), ("decompose_ms", c_double),
                 ("search_ms", c_double), ("wall_ms", c_double), ("spilled", c_uint64), ("pauses", c_uint64),
-                ("nlisted", c_uint64), ("list_off", POINTER(c_uint64)), ("list_ids", POINTER(c_uint32))]
+                ("nlisted", c_uint64), ("list_off", POINTER(c_uint64)), ("list_ids", POINTER(c_uint32)),
+                ("pn_first", c_uint32), ("status", ctypes.c_int32)]
 
 
 class KpTune(ctypes.Structure):
@@ -445,3 +446,37 @@ def batch_graph_specs(count=4096, nmax=20000, seed=3, kset=(2, 3, 4), qfun=lambd
 def graph_from_spec(s):
     return Graph.chung_lu(s["n"], s["gamma"], s["m_target"], s["max_deg"], s["ncomm"], s["cmin"], s["cmax"],
                           s["dens"], s["seed"])
+
+
+def served_graph_specs():
+    """The 256 resident graphs of the served workload: built like the many-graph
+    workload but with n log-uniform in [200, 200,000]."""
+    return batch_graph_specs(count=256, nmax=200000, seed=4, kset=(3, 4, 5), qfun=lambda k: 5 * k + 10)
+
+
+def served_queries(rate=200.0, duration_s=120.0, seed=7, ngraphs=256):
+    """The served query stream: Poisson arrivals at `rate` per second over
+    `duration_s` seconds; each query names a graph uniformly, k in {3,4,5} and
+    q = 5k+10.  Returns (arrival times in s, [(graph index, k, q)])."""
+    rng = np.random.default_rng(seed)
+    arrivals, queries = [], []
+    t = rng.exponential(1.0 / rate)
+    while t < duration_s:
+        k = int(rng.choice([3, 4, 5]))
+        arrivals.append(t)
+        queries.append((int(rng.integers(ngraphs)), k, 5 * k + 10))
+        t += rng.exponential(1.0 / rate)
+    return arrivals, queries
+
+
+def served_fixture_queries():
+    """The served queries with recorded oracle results: the first 32 of the
+    stream, plus each of the 8 largest graphs (n >= 1e5) at k = 3, 4, 5 in turn."""
+    specs = served_graph_specs()
+    _, qs = served_queries(duration_s=1.0)
+    out = list(qs[:32])
+    big = sorted(range(len(specs)), key=lambda i: -specs[i]["n"])[:8]
+    for j, gi in enumerate(big):
+        k = 3 + j % 3
+        out.append((gi, k, 5 * k + 10))
+    return out

@@ -2,10 +2,15 @@
 workloads the GPU parity tests compare against (run here, where the oracle
 has the time; the GPU box only reads the JSON).
 
-    python tests/golden/make_golden.py [--with-large]
+    python tests/golden/make_golden.py [--with-large] [--with-spill]
+    python tests/golden/make_golden.py --batch 48        # oracle_batch.json
+    python tests/golden/make_golden.py --served          # oracle_served.json
+    python tests/golden/make_golden.py --cl75k-chunks    # oracle_cl75k_chunks.json
 
 Each entry records count, 128-bit hash, canonical task count, valid roots,
 preprocessed sizes and the oracle's wall time on this container's cores.
+The batch, served and chunk files are written entry by entry, so an
+interrupted run resumes where it stopped.
 """
 import argparse
 import gzip
@@ -38,6 +43,87 @@ def run(og, k, q, truss, listing=False, window=None):
     return d
 
 
+def per_item(og, k, q, truss=0, window=None, threads=None):
+    t0 = time.time()
+    wb, we = window if window else (0, -1)
+    r = kpo.enumerate_plexes(og, k, q, truss, threads=threads or os.cpu_count(), pos_begin=wb, pos_end=we)
+    d = {x: r[x] for x in ITEM_KEYS}
+    d.update(k=k, q=q, oracle_s=round(time.time() - t0, 3))
+    return d
+
+
+ITEM_KEYS = ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l")
+
+
+def load_json(path):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def save_json(path, obj):
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(obj, f, indent=1)
+    os.replace(tmp, path)
+
+
+def batch_fixtures(ngraphs, threads):
+    """The first `ngraphs` graphs of the many-graph workload (kplex.batch_graph_specs)."""
+    path = os.path.join(ROOT, "tests", "golden", "oracle_batch.json")
+    gold = load_json(path)
+    items = gold.setdefault("items", {})
+    gold["spec"] = "kplex.batch_graph_specs()[:N]: graph i at its own k and q = 3k+8"
+    for i, s in enumerate(kplex.batch_graph_specs()[:ngraphs]):
+        if str(i) in items:
+            continue
+        og = from_graph(kplex.graph_from_spec(s))
+        items[str(i)] = per_item(og, s["k"], s["q"], threads=threads)
+        items[str(i)]["n"] = s["n"]
+        save_json(path, gold)
+        print("batch", i, items[str(i)], flush=True)
+
+
+def served_fixtures(threads):
+    """kplex.served_fixture_queries() over kplex.served_graph_specs()."""
+    path = os.path.join(ROOT, "tests", "golden", "oracle_served.json")
+    gold = load_json(path)
+    items = gold.setdefault("items", [])
+    gold["spec"] = "kplex.served_fixture_queries() over kplex.served_graph_specs(), in that order"
+    specs = kplex.served_graph_specs()
+    for j, (gi, k, q) in enumerate(kplex.served_fixture_queries()):
+        if j < len(items):
+            continue
+        og = from_graph(kplex.graph_from_spec(specs[gi]))
+        d = per_item(og, k, q, threads=threads)
+        d.update(graph=gi, n=specs[gi]["n"])
+        items.append(d)
+        save_json(path, gold)
+        print("served", j, d, flush=True)
+
+
+def cl75k_chunks(threads, step):
+    """Root windows of the Chung-Lu 75k workload past [0, 9000), `step` positions each."""
+    path = os.path.join(ROOT, "tests", "golden", "oracle_cl75k_chunks.json")
+    gold = load_json(path)
+    items = gold.setdefault("items", {})
+    g, k, q, _, tr = kplex.workload_cl75k()
+    og = from_graph(g)
+    pn = kpo.preprocess(og, k, q, tr)[0].shape[0]
+    end = pn - q + 2
+    gold["roots_total"] = end
+    b = 9000
+    while b < end:
+        e = min(end, b + step)
+        key = f"{b}_{e}"
+        if key not in items:
+            items[key] = per_item(og, k, q, tr, window=(b, e), threads=threads)
+            save_json(path, gold)
+            print("cl75k", key, items[key], flush=True)
+        b = e
+
+
 def from_graph(g):
     deg, nbr = g.csr()
     return kpo.OGraph.from_csr(deg, nbr)
@@ -49,7 +135,20 @@ def main():
                     help="also the Chung-Lu 75k root windows [4000,6000) and [0,9000) (minutes of oracle time)")
     ap.add_argument("--with-spill", action="store_true",
                     help="also the Chung-Lu 320k (truss) root window [0,800): 1.96e9 tasks (minutes of oracle time)")
+    ap.add_argument("--batch", type=int, default=0, help="oracle results of the first N many-graph members")
+    ap.add_argument("--served", action="store_true", help="oracle results of the served fixture queries")
+    ap.add_argument("--cl75k-chunks", action="store_true", help="Chung-Lu 75k root windows past 9000")
+    ap.add_argument("--chunk", type=int, default=250)
+    ap.add_argument("--threads", type=int, default=0)
     args = ap.parse_args()
+    if args.batch or args.served or args.cl75k_chunks:
+        if args.batch:
+            batch_fixtures(args.batch, args.threads)
+        if args.served:
+            served_fixtures(args.threads)
+        if args.cl75k_chunks:
+            cl75k_chunks(args.threads, args.chunk)
+        return
     gold = {}
     if os.path.exists(OUT):
         with open(OUT) as f:

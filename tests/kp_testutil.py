@@ -66,3 +66,16 @@ def csr(adj):
 
 def edges(adj):
     return [(u, v) for u in range(len(adj)) for v in adj[u] if u < v]
+
+
+def clique_plus_one(n):
+    """A clique on 0..n-1 plus vertex n adjacent to 0..n-2 (not to n-1).  For
+    any k with 2 <= k <= n and q <= n+1 the one maximal k-plex of size >= q is
+    the whole graph (every vertex misses at most one other), and the root that
+    finds it (vertex n, first in the degeneracy order) has vertex n-1 as its
+    only 2-hop vertex: the k-search must take it."""
+    adj = [set(range(n)) - {u} for u in range(n)]
+    for u in range(n - 1):
+        adj[u].add(n)
+    adj.append(set(range(n - 1)))
+    return adj

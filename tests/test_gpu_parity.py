@@ -10,7 +10,7 @@ import pytest
 import kplex
 import kpo
 from conftest import GOLDEN
-from kp_testutil import brute_force, csr, disjoint_cliques, multipartite, random_graph
+from kp_testutil import brute_force, clique_plus_one, csr, disjoint_cliques, multipartite, random_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -96,6 +96,15 @@ def test_size_classes(gsize, k, q):
     assert r["max_n"] >= gsize - 5
     same(r, o)
     assert [tuple(p) for p in r["plexes"]] == o["plexes"]
+
+
+def test_k_search_budget_above_256():
+    # k = 257 (budget 256) on a 601-vertex block: the one maximal plex needs
+    # the 2-hop vertex taken by the k-search
+    g = kplex.Graph.from_csr(*csr(clique_plus_one(600)))
+    r = gpu(g, 257, 514, listing=True, name="k257")
+    assert r["count"] == 1 and [tuple(p) for p in r["plexes"]] == [tuple(range(601))]
+    same(r, oracle(g, 257, 514))
 
 
 def test_er2000_set_diff(gold):

@@ -119,12 +119,39 @@ def test_server_query_larger_than_budget_is_refused():
     graphs, queries = graphs_and_queries()
     srv = kplex.Server(max_inflight=2, mem_budget_bytes=4096)
     gid = srv.add_graph(graphs[1])
-    srv.submit(gid, 2, 20)
-    with pytest.raises(kplex.KplexError) as e:
-        for _ in range(1000):
-            srv.poll()
-            time.sleep(0.001)
-    assert e.value.code == kplex.KP_ENOMEM
+    t = srv.submit(gid, 2, 20)
+    say("arena budget smaller than one query")
+    got = drain(srv, 1)
+    assert got[t]["status"] == kplex.KP_ENOMEM and got[t]["count"] == 0
+    assert srv.outstanding() == 0
+    srv.close()
+
+
+def test_server_failed_query_frees_its_slot_and_others_run():
+    # a clique of 1,100 vertices: its first root's block exceeds KP_MAX_BLOCK
+    # (KP_ERANGE).  Staged in one group with valid queries, it must leave with
+    # an error result while the others run to their plain results, and every
+    # slot must come back (the server drains).
+    graphs, queries = graphs_and_queries()
+    n = 1100
+    big = kplex.Graph.from_csr([n - 1] * n, [v for u in range(n) for v in range(n) if v != u])
+    srv = kplex.Server(max_inflight=3, slice_ms=0.5)
+    ids = [srv.add_graph(g) for g in graphs]
+    bid = srv.add_graph(big)
+    say("failing query in a group")
+    tb = srv.submit(bid, 2, 4)
+    ta = srv.submit(ids[2], 3, 14)
+    tc = srv.submit(ids[0], 3, 18)
+    got = drain(srv, 3)
+    assert got[tb]["status"] == kplex.KP_ERANGE and got[tb]["count"] == 0
+    for t, qy in ((ta, (2, 3, 14)), (tc, (0, 3, 18))):
+        assert got[t]["status"] == kplex.KP_OK
+        same(got[t], plain(graphs, qy))
+    # the slots are free again: a later query runs as well
+    td = srv.submit(ids[2], 2, 10)
+    got = drain(srv, 1)
+    same(got[td], plain(graphs, (2, 2, 10)))
+    assert srv.outstanding() == 0
     srv.close()
 
 

@@ -107,7 +107,119 @@ def test_preprocess_matches_oracle_datasets(datasets, truss):
     pn = compare_prep(g, 3, 12, truss)
     assert pn == (3156 if not truss else 1086)
     g = kplex.Graph.load_bin(datasets["soc-epinions"])
-    assert compare_prep(g, 4, 30, truss) == (4315 if not truss else compare_prep(g, 4, 30, truss))
+    assert compare_prep(g, 4, 30, truss) == (4315 if not truss else 622)
+
+
+def textbook_core_numbers(deg, nbr):
+    """Core number of every vertex by the bin-sort peel of Batagelj and
+    Zaversnik (2003): process vertices in nondecreasing current degree and
+    lower each unprocessed neighbour's degree by one.  The core numbers are
+    unique, so any correct peel gives the same numbers."""
+    n = len(deg)
+    off = np.concatenate(([0], np.cumsum(deg))).astype(np.int64)
+    d = [int(x) for x in deg]
+    md = max(d) if n else 0
+    bins = [0] * (md + 1)
+    for x in d:
+        bins[x] += 1
+    start = 0
+    for k in range(md + 1):
+        bins[k], start = start, start + bins[k]
+    pos, vert = [0] * n, [0] * n
+    for v in range(n):
+        pos[v] = bins[d[v]]
+        vert[pos[v]] = v
+        bins[d[v]] += 1
+    for k in range(md, 0, -1):
+        bins[k] = bins[k - 1]
+    if md >= 0 and bins:
+        bins[0] = 0
+    for i in range(n):
+        v = vert[i]
+        for u in nbr[off[v]:off[v + 1]].tolist():
+            if d[u] > d[v]:
+                du, pu = d[u], pos[u]
+                pw = bins[du]
+                w = vert[pw]
+                if u != w:
+                    pos[u], vert[pu] = pw, w
+                    pos[w], vert[pw] = pu, u
+                bins[du] += 1
+                d[u] -= 1
+    return np.array(d, dtype=np.int64)
+
+
+def check_degeneracy_order(deg, nbr, dseq):
+    """dseq is a minimum-degree peel of the graph (deg, nbr): every popped
+    vertex has the least number of unpopped neighbours among the unpopped
+    vertices, and the running maximum of those pop degrees is each vertex's
+    core number (computed independently above)."""
+    n = len(deg)
+    assert sorted(dseq.tolist()) == list(range(n))
+    off = np.concatenate(([0], np.cumsum(deg))).astype(np.int64)
+    rem = deg.astype(np.int64).copy()
+    alive = np.ones(n, dtype=bool)
+    core = np.zeros(n, dtype=np.int64)
+    run = 0
+    for u in dseq.tolist():
+        assert rem[u] == rem[alive].min(), u
+        run = max(run, int(rem[u]))
+        core[u] = run
+        alive[u] = False
+        rem[nbr[off[u]:off[u + 1]]] -= 1
+    assert np.array_equal(core, textbook_core_numbers(deg, nbr))
+
+
+@pytest.mark.parametrize("name,k,q,truss", [("email-euall", 3, 12, 0), ("email-euall", 3, 12, 1),
+                                            ("soc-epinions", 4, 30, 0), ("soc-epinions", 4, 30, 1)])
+def test_degeneracy_order_is_a_min_degree_peel(datasets, name, k, q, truss):
+    # Both implementations (the library's host preprocessing and the oracle)
+    # must produce a minimum-degree peel whose running maximum gives the
+    # textbook core numbers.
+    g = kplex.Graph.load_bin(datasets[name])
+    deg, nbr, _, dseq = kplex.preprocess(g, k, q, truss)
+    check_degeneracy_order(deg, nbr, dseq)
+    d0, n0 = g.csr()
+    odeg, onbr, _, odseq = kpo.preprocess(kpo.OGraph.from_csr(d0, n0), k, q, truss)
+    check_degeneracy_order(odeg, onbr, odseq)
+
+
+def test_min_degree_peel_check_catches_a_swapped_order():
+    # the check above fails for an order that is not a minimum-degree peel
+    adj = random_graph(120, 0.1, 3)
+    deg, nbr = csr(adj)
+    deg, nbr = np.array(deg, dtype=np.uint32), np.array(nbr, dtype=np.uint32)
+    g = kplex.Graph.from_csr(deg, nbr)
+    pdeg, pnbr, _, dseq = kplex.preprocess(g, 1, 2)
+    check_degeneracy_order(pdeg, pnbr, dseq)
+    bad = dseq.copy()
+    bad[0], bad[-1] = bad[-1], bad[0]
+    with pytest.raises(AssertionError):
+        check_degeneracy_order(pdeg, pnbr, bad)
+
+
+@pytest.mark.parametrize("impl", ["library", "oracle"])
+def test_degeneracy_tie_order_hand_derived(impl):
+    # The bucket heap of inc/degeneracy.h inserts at the head of a bucket and
+    # moves a decremented vertex to the head of its new bucket; pop takes the
+    # head of the lowest bucket.  Worked by hand:
+    #   cycle 0-1-2-3-4-0 (all degree 2): inserts leave bucket 2 = [4,3,2,1,0];
+    #   pop 4; its neighbours 0 then 3 move to bucket 1 -> [3,0]; pop 3; 2 moves
+    #   to the head -> [2,0]; pop 2; 1 -> [1,0]; pop 1; 0 -> bucket 0; pop 0.
+    #   path 0-1-2-3: bucket 1 = [3,0], bucket 2 = [2,1]; pop 3; 2 -> bucket 1
+    #   head [2,0]; pop 2; 1 -> [1,0]; pop 1; 0 -> bucket 0; pop 0.
+    # An insert or a decrement at a bucket's tail gives another order.
+    cases = [(5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)], [4, 3, 2, 1, 0]),
+             (4, [(0, 1), (1, 2), (2, 3)], [3, 2, 1, 0])]
+    for n, edges, want in cases:
+        g = kplex.Graph.from_edges(n, edges)
+        if impl == "library":
+            _, _, orig, dseq = kplex.preprocess(g, 1, 2)
+        else:
+            d0, n0 = g.csr()
+            _, _, orig, dseq = kpo.preprocess(kpo.OGraph.from_csr(d0, n0), 1, 2)
+        assert orig.tolist() == list(range(n))
+        assert dseq.tolist() == want
 
 
 @pytest.mark.parametrize("seed", range(5))
@@ -159,20 +271,3 @@ def test_server_needs_a_device():
     with pytest.raises(kplex.KplexError) as e:
         kplex.Server()
     assert e.value.code == kplex.KP_ENODEV
-
-
-def _hw_queues_after_load(env):
-    code = ("import ctypes, sys; sys.path.insert(0, sys.argv[1]); import kplex; kplex.load(); "
-            "libc = ctypes.CDLL(None); libc.getenv.restype = ctypes.c_char_p; "
-            "v = libc.getenv(b'GPU_MAX_HW_QUEUES'); print(v.decode() if v else '')")
-    out = subprocess.run([sys.executable, "-c", code, os.path.join(ROOT, "maximal-kplex-enumeration_amd")],
-                         env=env, capture_output=True, text=True, timeout=120)
-    assert out.returncode == 0, out.stderr
-    return out.stdout.strip().splitlines()[-1] if out.stdout.strip() else ""
-
-
-def test_library_requests_hw_queues_only_when_unset():
-    env = {k: v for k, v in os.environ.items() if k != "GPU_MAX_HW_QUEUES"}
-    assert _hw_queues_after_load(env) == "16"
-    env["GPU_MAX_HW_QUEUES"] = "4"
-    assert _hw_queues_after_load(env) == "4"

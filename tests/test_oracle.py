@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import kpo
-from kp_testutil import brute_force, csr, disjoint_cliques, multipartite, random_graph
+from kp_testutil import brute_force, clique_plus_one, csr, disjoint_cliques, multipartite, random_graph
 from conftest import GOLDEN
 
 
@@ -126,3 +126,9 @@ def test_golden_fixtures_match_oracle(datasets):
     r = kpo.enumerate_plexes(kpo.OGraph.from_csr(deg, nbr), k, q, truss, listing=True)
     assert (r["count"], r["hash"], r["tasks"]) == (g["count"], g["hash"], g["tasks"])
     assert [list(p) for p in r["plexes"]] == g["plexes"]
+
+
+def test_k_search_budget_above_256():
+    # k = 257: the k-search budget k-1 = 256 must not wrap
+    r = kpo.enumerate_plexes(ograph(clique_plus_one(600)), 257, 514, listing=True, threads=4)
+    assert r["count"] == 1 and r["plexes"] == [tuple(range(601))]
