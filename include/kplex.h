@@ -126,6 +126,7 @@ typedef struct kp_result {
 int  kp_enumerate(const kp_graph *g, const kp_params *p, kp_result *r);
 void kp_result_free(kp_result *r);
 
+
 /* Host preprocessing only (no device needed): the (q-k)-core after the
  * optional truss (inc/host_funcs.h:749-823), its new->original id map and the
  * degeneracy order (inc/host_funcs.h:829-851).  out receives the preprocessed

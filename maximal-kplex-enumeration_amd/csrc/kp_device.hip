@@ -48,17 +48,6 @@ constexpr uint32_t CLAIM = 0x40000000u;
 constexpr uint32_t CMASK = 0x3FFFFFFFu;
 constexpr uint32_t SPIN_LIMIT = 1u << 24;
 
-// One shard of the frame queue: a bounded MPMC ring with sequence-numbered
-// slots.  Shards sit on separate 128-byte lines so that waves polling and
-// claiming different shards do not contend on one address.
-struct SubQ {
-    uint32_t count;        // published frames
-    uint32_t size;         // reserved slots
-    uint64_t head, tail;   // tickets
-    uint32_t pad[26];
-};
-static_assert(sizeof(SubQ) == 128, "SubQ is one 128-byte line");
-constexpr uint32_t NSUBQ = 32;
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ uint64_t lt_mask() { return (1ull << __lane_id()) - 1ull; }
@@ -124,18 +113,6 @@ __device__ __forceinline__ void fence_agent() { __builtin_amdgcn_fence(__ATOMIC_
 __device__ __forceinline__ void fence_wg() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
 
 // ============================================================ decompose
-struct DecompArgs {
-    const JobDev *jobs;
-    const uint32_t *slot_job, *slot_pos;   // per slot: job index and root position
-    uint32_t nslots;
-    uint32_t *blk, *lft;
-    uint32_t lcap;
-    RootInfo *info;
-    uint32_t *ws_cnt, *ws_b1c;
-    uint64_t *ws_a1;
-    uint32_t cnt_stride;                   // counter words per wave (largest pn)
-    uint32_t d1cap, w1cap, nwaves;
-};
 
 // Root r = dseq[pos] (src/device_funcs.cu:28-298), restated on bitsets:
 //  B1 = (q-2k)-core of G[later neighbours]   (fixpoint :139-171; the core is
@@ -144,24 +121,43 @@ struct DecompArgs {
 //  B2 = later non-neighbours with >= q-2k+2 neighbours in B1        (:211-273)
 //  L2 = earlier non-neighbours with >= q-2k+3 neighbours in B1
 // Canonical layout: B = [r, B1 asc, B2 asc], L = [L1 asc, L2].
+// Per-wave workspace of the decomposition: the counter array over the job's
+// vertices (zero between roots), the later-neighbour list and its bitset
+// adjacency in HBM, three LDS scratch rows.
+struct DecompWs {
+    uint32_t *cnt, *b1c;
+    uint64_t *A1;
+    uint64_t *row, *alive;
+    uint32_t *sb2;
+};
+
+// Decomposes the root at degeneracy position pos of job J into blk/lft at
+// `slot` and writes info[slot]; the whole wave calls it.
+__device__ void decompose_root(const DecompArgs &a, const JobDev &J, int32_t pos, uint32_t slot, const DecompWs &ws);
+
 __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
 {
     __shared__ uint64_t s_row[WPB][64];
     __shared__ uint64_t s_alive[WPB][64];
     __shared__ uint32_t s_b2[WPB][NCAP];
-    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
     const uint32_t gw = blockIdx.x * WPB + wv;
     if (gw >= a.nwaves) return;
-    uint32_t *cnt = a.ws_cnt + size_t(gw) * a.cnt_stride;
-    uint32_t *b1c = a.ws_b1c + size_t(gw) * a.d1cap;
-    uint64_t *A1 = a.ws_a1 + size_t(gw) * a.d1cap * a.w1cap;
-    uint64_t *row = s_row[wv], *alive = s_alive[wv];
-    uint32_t *sb2 = s_b2[wv];
-
+    const DecompWs ws{a.ws_cnt + size_t(gw) * a.cnt_stride, a.ws_b1c + size_t(gw) * a.d1cap,
+                      a.ws_a1 + size_t(gw) * a.d1cap * a.w1cap, s_row[wv], s_alive[wv], s_b2[wv]};
     for (uint32_t slot = gw; slot < a.nslots; slot += a.nwaves) {
         const JobDev J = a.jobs[rfl(a.slot_job[slot])];
+        decompose_root(a, J, int32_t(rfl(a.slot_pos[slot])), slot, ws);
+    }
+}
+
+__device__ void decompose_root(const DecompArgs &a, const JobDev &J, int32_t pos, uint32_t slot, const DecompWs &ws)
+{
+    const int ln = __lane_id();
+    uint32_t *cnt = ws.cnt, *b1c = ws.b1c, *sb2 = ws.sb2;
+    uint64_t *A1 = ws.A1, *row = ws.row, *alive = ws.alive;
+    {
         const int lb2k = J.q - 2 * J.k, bd = J.q - J.k;
-        const int32_t pos = int32_t(rfl(a.slot_pos[slot]));
         const uint32_t r = uint32_t(J.dseq[pos]);
         const uint64_t ro = J.off[r];
         const uint32_t dr = J.deg[r];
@@ -194,12 +190,12 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
         info.wr += rfl64(wsum64(wsum));
         if (int(m1) < bd) {   // |B| - 1 < q - k before any filtering
             if (ln == 0) a.info[slot] = info;
-            continue;
+            return;
         }
         if (m1 > a.d1cap || m1 > 64u * 64u) {
             info.status = -1;
             if (ln == 0) a.info[slot] = info;
-            continue;
+            return;
         }
         const uint32_t w1 = (m1 + 63) >> 6;
 
@@ -258,13 +254,13 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
         for (uint32_t t = 0; t < w1; t++) nb1 += __popcll(alive[t]);
         if (int(nb1) < bd) {
             if (ln == 0) a.info[slot] = info;
-            continue;
+            return;
         }
         const uint32_t hop = nb1 + 1;
         if (hop > NCAP) {
             info.status = -1;
             if (ln == 0) a.info[slot] = info;
-            continue;
+            return;
         }
         if (ln == 0) B[0] = r;
         {
@@ -368,16 +364,6 @@ __global__ __launch_bounds__(256) void decompose_kernel(DecompArgs a)
 }
 
 // ============================================================ build
-struct BuildArgs {
-    const JobDev *jobs;
-    const uint32_t *blk, *lft;
-    uint32_t lcap;
-    const RootDesc *desc;
-    uint32_t ndesc;
-    uint32_t *ws_cnt;
-    uint32_t cnt_stride, nwaves;
-    uint64_t *wr_extra;   // per desc: sum of degrees over B and L
-};
 
 // Per root: adjacency rows adj[i] (bit j = B[j] adjacent to B[i]), class rows
 // cls2[i] (bit j: class(i,j) >= UNLINK2EQUAL) and cls4[i] (>= UNLINK2MORE)
@@ -385,18 +371,31 @@ struct BuildArgs {
 // common = |N(i) ∩ N(j) ∩ B[0,h)|; left rows ladj[u] (bit i = L[u] adjacent
 // to B[i]); original ids.  Row stride is the class width WC.
 template <int WC>
+__device__ void build_root(const BuildArgs &a, const RootDesc &d, const JobDev &J, uint32_t di, uint32_t *cnt,
+                           uint64_t *row);
+
+template <int WC>
 __global__ __launch_bounds__(256) void build_kernel(BuildArgs a)
 {
     __shared__ uint64_t s_row[WPB][WC];
-    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
     const uint32_t gw = blockIdx.x * WPB + wv;
     if (gw >= a.nwaves) return;
     uint32_t *cnt = a.ws_cnt + size_t(gw) * a.cnt_stride;
-    uint64_t *row = s_row[wv];
-
     for (uint32_t di = gw; di < a.ndesc; di += a.nwaves) {
         const RootDesc d = a.desc[di];
-        const JobDev J = a.jobs[d.job];
+        build_root<WC>(a, d, a.jobs[d.job], di, cnt, s_row[wv]);
+    }
+}
+
+// Writes root d's rows into its arena (d.addr) and a.wr_extra[di]; the whole
+// wave calls it, with a zeroed counter array over the job's vertices.
+template <int WC>
+__device__ void build_root(const BuildArgs &a, const RootDesc &d, const JobDev &J, uint32_t di, uint32_t *cnt,
+                           uint64_t *row)
+{
+    const int ln = __lane_id();
+    {
         const int k = d.k, q = d.q;
         const int km2 = max(k - 2, 0), km3 = max(k - 3, 0);
         const int thPP1 = q - k - 2 * km2, thPP2 = q - k - 2 * km3;
@@ -501,32 +500,6 @@ __global__ __launch_bounds__(256) void build_kernel(BuildArgs a)
 }
 
 // ============================================================ search
-struct SearchArgs {
-    const RootDesc *desc;      // class root table
-    const uint32_t *todo;      // roots to start, in order (indices into desc)
-    uint32_t ntodo;
-    const uint64_t *seed;      // frames to resume
-    uint32_t nseed;
-    uint64_t *dump;            // frames written on pause
-    uint32_t dump_cap;
-    JobAcc *acc;               // per-job accumulators
-    uint64_t *qframes, *qseq;  // NSUBQ * qcap slots
-    uint32_t qcap;             // slots per shard
-    SubQ *subq;
-    Sched *sched;
-    uint32_t *list;
-    uint64_t list_cap;
-    uint64_t *list_used;       // shared by all size-class launches
-    uint32_t nwaves;
-    uint32_t ring_cap;         // frames per wave in the LDS ring
-    uint32_t back_cap;         // frames per wave in the HBM backing stack
-    uint64_t *gstack;          // nwaves * back_cap frames
-    uint32_t donate_period, donate_target;
-    uint64_t limit_ticks;      // 0 = no limit; else wall-clock ticks per launch
-    const uint32_t *host_pause;   // mapped host word (nonzero = pause) or null
-    uint64_t pause_frames;     // 0 = none; else pause once this many frames were processed
-    unsigned long long *t0;    // shared start tick of the epoch (0 until the first wave of any class starts)
-};
 
 enum : uint32_t { FT_BRANCH = 0, FT_KSEARCH = 1 };
 
@@ -543,7 +516,6 @@ struct FrameLayout {
     static constexpr int FW = (4 * W + 2) & ~1;
 };
 
-constexpr int FrameWords(int W) { return (4 * W + 2) & ~1; }
 
 template <int W>
 __device__ __forceinline__ uint64_t rdl64(uint64_t x, uint32_t l)
@@ -559,6 +531,7 @@ __device__ __forceinline__ uint64_t rdl64(uint64_t x, uint32_t l)
 struct Cold {
     uint64_t t_start, t_mark, busy_t, idle_t;
     uint64_t alg, task0, bpt, frames, pops;
+    uint64_t hungry_since;   // wall-clock tick at which the wave last became hungry
     uint32_t cjob, check_every;
 };
 
@@ -579,10 +552,19 @@ struct Worker {
     uint32_t cur = 0xffffffffu, n = 0, h = 0, l = 0;
     Cold *cold;
     const uint64_t *LA = nullptr;
-    // rows of the vertices this lane owns (64j + lane), kept in VGPRs
-    uint64_t adjR[W][W], c2R[W][W], c4R[W][W];
+    // Rows of the vertices this lane owns (64j + lane) are kept in VGPRs.  The
+    // search reads one class row per include step (the pivot's, a uniform
+    // row), so beyond two words per row the class rows, the pivot's adjacency
+    // row and the original ids stay in the root's arena and uniform rows come
+    // in through scalar loads: the VGPRs then hold only the adjacency rows
+    // that the per-vertex degrees need.
+    static constexpr bool kRowRegs = W <= 2;
+    static constexpr int RW = kRowRegs ? W : 1;
+    uint64_t adjR[W][W], c2R[RW][RW], c4R[RW][RW];
     uint64_t lftR[LW > 0 ? LW : 1][W];
-    uint32_t origR[W];
+    uint32_t origR[RW];
+    const uint64_t *Ab = nullptr, *C2b = nullptr, *C4b = nullptr;   // the root's arena rows (uniform)
+    const uint32_t *OG = nullptr;
     // accumulators (uniform); the bytes of the tasks are kept in Cold
     uint64_t cnt = 0, ha = 0, hb = 0, tasks = 0;
     uint32_t spills = 0;
@@ -610,6 +592,7 @@ struct Worker {
         cold->t_mark = now;
         cold->busy_t = cold->idle_t = 0;
         cold->alg = cold->task0 = cold->bpt = cold->frames = cold->pops = 0;
+        cold->hungry_since = 0;
         cold->cjob = 0xffffffffu;
         // frames between checks of the error flag, the deadline and the pause
         // sources; a frame budget shortens the interval so that small budgets
@@ -636,6 +619,12 @@ struct Worker {
         k = int(rfl(uint32_t(k)));
         q = int(rfl(uint32_t(q)));
         LA = reinterpret_cast<const uint64_t *>(rfl64(reinterpret_cast<uint64_t>(LA)));
+        if constexpr (!kRowRegs) {
+            Ab = reinterpret_cast<const uint64_t *>(rfl64(reinterpret_cast<uint64_t>(Ab)));
+            C2b = reinterpret_cast<const uint64_t *>(rfl64(reinterpret_cast<uint64_t>(C2b)));
+            C4b = reinterpret_cast<const uint64_t *>(rfl64(reinterpret_cast<uint64_t>(C4b)));
+            OG = reinterpret_cast<const uint32_t *>(rfl64(reinterpret_cast<uint64_t>(OG)));
+        }
         cnt = rfl64(cnt);
         ha = rfl64(ha);
         hb = rfl64(hb);
@@ -988,6 +977,10 @@ struct Worker {
         const uint64_t *C4 = C2 + size_t(n) * W;
         LA = C4 + size_t(n) * W;
         const uint32_t *og = reinterpret_cast<const uint32_t *>(LA + size_t(l) * W);
+        Ab = A;
+        C2b = C2;
+        C4b = C4;
+        OG = og;
         // Lanes past the end load the last row and zero it: no lane-dependent
         // branch, so the caller's join after a root switch stays uniform.
 #pragma unroll
@@ -997,13 +990,19 @@ struct Worker {
             const size_t vv = size_t(min(v, n - 1));
 #pragma unroll
             for (int w = 0; w < W; w++) {
-                const uint64_t x = A[vv * W + w], y = C2[vv * W + w], z = C4[vv * W + w];
+                const uint64_t x = A[vv * W + w];
                 adjR[j][w] = in ? x : 0ull;
-                c2R[j][w] = in ? y : 0ull;
-                c4R[j][w] = in ? z : 0ull;
             }
-            const uint32_t o = og[vv];
-            origR[j] = in ? o : 0u;
+            if constexpr (kRowRegs) {
+#pragma unroll
+                for (int w = 0; w < W; w++) {
+                    const uint64_t y = C2[vv * W + w], z = C4[vv * W + w];
+                    c2R[j][w] = in ? y : 0ull;
+                    c4R[j][w] = in ? z : 0ull;
+                }
+                const uint32_t o = og[vv];
+                origR[j] = in ? o : 0u;
+            }
         }
 #pragma unroll
         for (int t = 0; t < LW; t++) {
@@ -1020,15 +1019,46 @@ struct Worker {
     }
 
     // row of vertex v (uniform) out of the owner lane's registers
-    __device__ __forceinline__ void row_of(const uint64_t (&R)[W][W], uint32_t v, uint64_t *out) const
+    template <int RJ>
+    __device__ __forceinline__ void row_of(const uint64_t (&R)[RJ][RJ], uint32_t v, uint64_t *out) const
     {
         const uint32_t pw = v >> 6, pl = v & 63;
 #pragma unroll
-        for (int j = 0; j < W; j++)
+        for (int j = 0; j < RJ; j++)
             if (uint32_t(j) == pw) {
 #pragma unroll
-                for (int w = 0; w < W; w++) out[w] = rdl64<W>(R[j][w], pl);
+                for (int w = 0; w < RJ; w++) out[w] = rdl64<W>(R[j][w], pl);
             }
+    }
+    // row of vertex v (uniform) of an arena matrix, through scalar loads (the
+    // arena is read-only while the search kernel runs)
+    typedef const __attribute__((address_space(4))) uint64_t *ConstRow;
+    __device__ __forceinline__ void row_mem(const uint64_t *base, uint32_t v, uint64_t *out) const
+    {
+        ConstRow p = (ConstRow)(base + size_t(v) * W);
+#pragma unroll
+        for (int w = 0; w < W; w++) out[w] = p[w];
+    }
+    __device__ __forceinline__ void adj_row(uint32_t v, uint64_t *out) const
+    {
+        if constexpr (kRowRegs) row_of(adjR, v, out);
+        else row_mem(Ab, v, out);
+    }
+    __device__ __forceinline__ void cls2_row(uint32_t v, uint64_t *out) const
+    {
+        if constexpr (kRowRegs) row_of(c2R, v, out);
+        else row_mem(C2b, v, out);
+    }
+    __device__ __forceinline__ void cls4_row(uint32_t v, uint64_t *out) const
+    {
+        if constexpr (kRowRegs) row_of(c4R, v, out);
+        else row_mem(C4b, v, out);
+    }
+    // original id of vertex 64j + lane
+    __device__ __forceinline__ uint32_t orig_of(int j) const
+    {
+        if constexpr (kRowRegs) return origR[j];
+        else return OG[min(64u * j + ln, n - 1)];
     }
 
     // ---- per-vertex helpers ----
@@ -1128,8 +1158,9 @@ struct Worker {
 #pragma unroll
         for (int j = 0; j < W; j++)
             if (has(Sx, j)) {
-                sa += mix64(uint64_t(origR[j]) ^ kp::HASH_SA);
-                sb += mix64(uint64_t(origR[j]) ^ kp::HASH_SB);
+                const uint64_t o = orig_of(j);
+                sa += mix64(o ^ kp::HASH_SA);
+                sb += mix64(o ^ kp::HASH_SB);
             }
         sa = rfl64(wsum64(sa));
         sb = rfl64(wsum64(sb));
@@ -1151,7 +1182,7 @@ struct Worker {
             uint64_t o = base + 2;
 #pragma unroll
             for (int j = 0; j < W; j++) {
-                if (has(Sx, j)) a.list[o + __popcll(Sx[j] & lt_mask())] = origR[j];
+                if (has(Sx, j)) a.list[o + __popcll(Sx[j] & lt_mask())] = orig_of(j);
                 o += __popcll(Sx[j]);
             }
             asm volatile("");   // the per-lane stores join here, before any other path merges
@@ -1192,7 +1223,7 @@ struct Worker {
         uint32_t pivot;
         if (minP + k < int(ps + cs)) {
             uint64_t adjPP[W];
-            row_of(adjR, pp, adjPP);
+            adj_row(pp, adjPP);
             uint32_t k2 = ~0u;
 #pragma unroll
             for (int j = 0; j < W; j++)
@@ -1242,7 +1273,7 @@ struct Worker {
 #pragma unroll
         for (int w = 0; w < W; w++) P2[w] = P[w] | (w == int(pw) ? pb : 0ull);
         compat_mask(P2, ps + 1, OK);
-        row_of(c2R, pivot, c2v);
+        cls2_row(pivot, c2v);
         uint64_t C2[W], PC2[W];
 #pragma unroll
         for (int w = 0; w < W; w++) {
@@ -1250,7 +1281,7 @@ struct Worker {
             PC2[w] = P2[w] | C2[w];
         }
         if (!ub_ok(P2, PC2)) return false;
-        row_of(c4R, pivot, c4v);
+        cls4_row(pivot, c4v);
 #pragma unroll
         for (int w = 0; w < W; w++) {
             nf.P[w] = P2[w];
@@ -1317,7 +1348,7 @@ struct Worker {
         }
         if (!push(make_meta(f.root, FT_KSEARCH, f.r), f.P, f.C, X1, D1, true)) return false;
         uint64_t c2u[W];
-        row_of(c2R, u, c2u);
+        cls2_row(u, c2u);
 #pragma unroll
         for (int w = 0; w < W; w++) {
             nf.P[w] = f.P[w] | (w == int(uw) ? ub : 0ull);
@@ -1418,6 +1449,53 @@ struct Worker {
         rn = bn = 0;
     }
 
+    // Drops the accumulators of the root in progress (its subtree goes to
+    // another path, which counts it from the start).
+    __device__ void discard()
+    {
+        cnt = ha = hb = tasks = 0;
+        cold->alg = 0;
+        cold->task0 = 0;
+        rn = bn = 0;
+    }
+
+    // Monolithic per-root mode: this wave walks the whole subtree of root
+    // ridx on its own stack, with no queue, donation or pause.  Returns true
+    // when the subtree is done (accumulators flushed to the root's job), false
+    // once more than `budget` frames were processed (0 = no budget) or on an
+    // error; then the root's accumulators are dropped.
+    __device__ bool run_single(uint32_t ridx, uint64_t budget)
+    {
+        cur = 0xffffffffu;   // the descriptor slot is reused root after root
+        if (past_deadline()) set_error(KP_ETIMEDOUT);
+        if (err) return false;
+        push_root(ridx);
+        uint64_t frames = 0;
+        Frame<W> f;
+        while (depth() > 0 && !err) {
+            launder();
+            pop_top(f);
+            for (;;) {
+                if (f.root != cur) load_root<true>(f.root);
+                Frame<W> nf;
+                const bool more = (f.type == FT_KSEARCH) ? ksearch(f, nf) : branch(f, nf);
+                frames++;
+                if ((frames & 1023u) == 0 && past_deadline()) set_error(KP_ETIMEDOUT);
+                if (err || !more) break;
+                f = nf;
+                if (budget && frames > budget) break;
+            }
+            if (budget && frames > budget) break;
+        }
+        if (err || (budget && frames > budget)) {
+            discard();
+            return false;
+        }
+        close_alg();
+        flush();
+        return true;
+    }
+
     // Termination: a wave counts itself in S->active only while it holds
     // frames, and it increments S->active before taking a frame, a seed or a
     // root (seq_cst), so "no seeds or roots left, queue empty, nobody
@@ -1511,6 +1589,7 @@ struct Worker {
                         if (!hungry_self) {
                             if (lane0()) atomicAdd(&S->hungry, 1u);
                             hungry_self = true;
+                            cold->hungry_since = wall_clock64();
                         }
                         uint32_t done = 0, stop = 0;
                         if (lane0()) {
@@ -1536,6 +1615,35 @@ struct Worker {
                         if (pause_requested(0)) {
                             paused = true;
                             break;
+                        }
+                        // Retirement: once the class has nothing left to start,
+                        // a wave that has stayed hungry for retire_ticks while
+                        // hungry waves outnumber the busy ones by more than a
+                        // margin exits, so that the CU slots it holds go to
+                        // other size classes whose blocks wait for them.  While
+                        // busy waves hold large subtrees their donations feed
+                        // hungry waves within a millisecond or so, so only a
+                        // class whose remaining work is thin sheds waves; the
+                        // margin keeps enough hungry waves for donations to
+                        // double the busy set each round.
+                        if (backoff >= 16 && ns >= a.nseed && nr >= a.ntodo &&
+                            wall_clock64() - rfl64(cold->hungry_since) > a.retire_ticks) {
+                            uint32_t retire = 0;
+                            if (lane0()) {
+                                const uint32_t ac = __hip_atomic_load(&S->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                const uint32_t qc2 = __hip_atomic_load(&S->q_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                const uint32_t old = atomicSub(&S->hungry, 1u);
+                                if (qc2 == 0 && old > ac + a.retire_margin + 1u) {
+                                    retire = 1;
+                                    atomicAdd(&S->retired, 1u);
+                                } else {
+                                    atomicAdd(&S->hungry, 1u);
+                                }
+                            }
+                            if (rfl(retire)) {
+                                hungry_self = false;
+                                break;
+                            }
                         }
                         // jittered exponential backoff: waves do not poll in lockstep
                         const uint32_t jit = (gw * 2654435761u + backoff * 97u) >> 28;
@@ -1666,11 +1774,12 @@ struct Worker {
     }
 };
 
-// The one-word class runs at 4 waves per SIMD (<= 128 VGPRs, no scratch).
-// The two-word class stays at its natural 2: capped at 3 it fits 168 VGPRs
-// without scratch but measured slower (soc-epinions 470 ms against 355 ms).
+// Waves per SIMD (list-free kernels, compiler report): one word 7 (68
+// VGPRs; asked for at least 4), two to four words 4 (104-128 VGPRs), six
+// words 2 (189), eight words 2 when asked for (235, no scratch; 1 otherwise,
+// with AGPR spills); sixteen words 1 with 724 bytes of scratch per lane.
 template <int W, int LW, bool LISTS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1 ? 4 : 1))) void search_kernel(SearchArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1 ? 4 : (W == 8 ? 2 : 1)))) void search_kernel(SearchArgs a)
 {
     extern __shared__ uint64_t lds[];
     __shared__ Cold s_cold[4];
@@ -1687,8 +1796,90 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1 ? 4 
     wk.run();
 }
 
+
+// ============================================================ per-root
+// Monolithic per-root kernel: one wave per root decomposes it, builds its
+// subgraph into the wave's own arena slot and walks the whole subtree on its
+// own stack (Worker::run_single): one launch, no queue, no donation, no host
+// step between the phases.  [reference: the per-root kSearch3/BNB3 path,
+// src/device_funcs.cu:3027-4403, its "naive root-based" form]  Roots whose
+// block is wider than the kernel's W words, or whose subtree passes the frame
+// budget, are listed in `deferred` for a wider kernel or the pooled path.
+template <int W, int LW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1 ? 4 : 1))) void root_kernel(RootKernArgs a)
+{
+    __shared__ uint64_t s_row[WPB][64];
+    __shared__ uint64_t s_alive[WPB][64];
+    __shared__ uint32_t s_b2[WPB][NCAP];
+    __shared__ Cold s_cold[WPB];
+    extern __shared__ uint64_t lds[];
+    constexpr int FW = FrameLayout<W>::FW;
+    const int wv = int(rfl(threadIdx.x >> 6));
+    const uint32_t gw = blockIdx.x * WPB + wv;
+    if (gw >= a.d.nwaves) return;
+    const int ln = __lane_id();
+    const DecompWs ws{a.d.ws_cnt + size_t(gw) * a.d.cnt_stride, a.d.ws_b1c + size_t(gw) * a.d.d1cap,
+                      a.d.ws_a1 + size_t(gw) * a.d.d1cap * a.d.w1cap, s_row[wv], s_alive[wv], s_b2[wv]};
+    uint64_t *ring = lds + size_t(wv) * a.s.ring_cap * FW;
+    uint64_t *back = a.s.gstack ? a.s.gstack + size_t(gw) * a.s.back_cap * FW : nullptr;
+    Worker<W, LW, false> wk(a.s, ring, back, &s_cold[wv]);
+    uint64_t *arena = a.arena + size_t(gw) * a.arena_words;
+    for (uint32_t slot = gw; slot < a.d.nslots; slot += a.d.nwaves) {
+        const uint32_t job = rfl(a.d.slot_job[slot]);
+        const JobDev J = a.d.jobs[job];
+        decompose_root(a.d, J, int32_t(rfl(a.d.slot_pos[slot])), gw, ws);
+        fence_agent();
+        RootInfo ri = a.d.info[gw];
+        ri.status = int32_t(rfl(uint32_t(ri.status)));
+        ri.n = rfl(ri.n);
+        ri.h = rfl(ri.h);
+        ri.l = rfl(ri.l);
+        ri.wr = rfl64(ri.wr);
+        if (ri.status < 0) {
+            atomicCAS(&a.s.sched->error, 0, ri.status == -1 ? KP_ERANGE : KP_EOVERFLOW);
+            break;
+        }
+        if (ri.status != 1) continue;
+        if (ri.n > 64u * W) {
+            if (ln == 0) a.deferred[atomicAdd(a.ndeferred, 1u)] = slot;
+            continue;
+        }
+        RootDesc d{};
+        d.addr = reinterpret_cast<uint64_t>(arena);
+        d.n = ri.n;
+        d.h = ri.h;
+        d.l = ri.l;
+        d.slot = gw;
+        d.job = job;
+        d.k = uint16_t(J.k);
+        d.q = uint16_t(J.q);
+        const uint64_t words = 3ull * ri.n * W + uint64_t(ri.l) * W + (ri.n + 1) / 2;
+        for (uint64_t t = ln; t < words; t += 64) arena[t] = 0;
+        fence_agent();
+        build_root<W>(a.b, d, J, gw, ws.cnt, s_row[wv]);
+        if (ln == 0) a.desc[gw] = d;
+        fence_agent();
+        if (wk.run_single(gw, a.budget)) {
+            const uint64_t w = ri.wr + rfl64(a.b.wr_extra[gw]);
+            const uint64_t n = ri.n;
+            if (ln == 0) {
+                JobAcc *A = a.s.acc + job;
+                atomicAdd(&A->roots, 1ull);
+                atomicAdd(&A->adj, (unsigned long long)(4 * w));
+                atomicAdd(&A->dalg, (unsigned long long)(4 * w + (n * n + 7) / 8 + n * n));
+                atomicMax(&A->max_n, ri.n);
+                atomicMax(&A->max_l, ri.l);
+            }
+        } else {
+            if (wk.err) break;
+            if (ln == 0) a.deferred[atomicAdd(a.ndeferred, 1u)] = slot;
+        }
+    }
+}
+
 // fills n words with v (a kernel rather than hipMemsetAsync, so that the
-// launch sequence stays capturable as a graph)
+// launch sequence stays capturable as a graph and needs no runtime call that
+// may allocate)
 __global__ void fill_u64_kernel(uint64_t *p, uint32_t n, uint64_t v)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1756,6 +1947,11 @@ void DevBuf::release()
 namespace {
 
 size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// per-root pass of plain runs: graphs of at most kMonoMaxPn preprocessed
+// vertices; a root whose subtree passes kMonoBudget frames goes to the pool
+constexpr uint32_t kMonoMaxPn = 4096;
+constexpr uint64_t kMonoBudget = 4096;
 
 // the reference's task-queue capacity MAX_CAP = 2048 * 2048 (inc/common.h:11,
 // the overflow queue of inc/host_funcs.h:982), the default for
@@ -1839,6 +2035,7 @@ namespace {
 struct DeviceStreams {
     bool ready = false;
     int cus = 0;
+    uint64_t ticks_per_ms = 100000;
     hipStream_t cls[kNumClass] = {};
     hipStream_t main = nullptr, side = nullptr;
 };
@@ -1854,6 +2051,9 @@ int device_streams(int device, DeviceStreams **out)
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, device));
         d.cus = prop.multiProcessorCount;
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+            d.ticks_per_ms = uint64_t(khz);
         for (int c = 0; c < kNumClass; c++) HIPCHK(hipStreamCreateWithFlags(&d.cls[c], hipStreamNonBlocking));
         HIPCHK(hipStreamCreateWithFlags(&d.main, hipStreamNonBlocking));
         HIPCHK(hipStreamCreateWithFlags(&d.side, hipStreamNonBlocking));
@@ -1877,6 +2077,7 @@ int Engine::init()
     int rc = device_streams(device, &ds);
     if (rc) return rc;
     cus = ds->cus;
+    ticks_per_ms = ds->ticks_per_ms;
     stream = ds->main;
     side = ds->side;
     for (int c = 0; c < kNumClass; c++) {
@@ -1933,20 +2134,12 @@ int Engine::zero_acc(uint32_t first, uint32_t count)
     return KP_OK;
 }
 
-int Engine::read_acc(uint32_t first, uint32_t count, std::vector<uint64_t> &out5)
+int Engine::read_acc(uint32_t first, uint32_t count, std::vector<JobAcc> &out)
 {
-    std::vector<JobAcc> h(count);
-    out5.assign(size_t(count) * 5, 0);
+    out.assign(count, JobAcc{});
     if (!count) return KP_OK;
-    HIPCHK(hipMemcpyAsync(h.data(), acc.as<JobAcc>() + first, sizeof(JobAcc) * count, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(out.data(), acc.as<JobAcc>() + first, sizeof(JobAcc) * count, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    for (uint32_t i = 0; i < count; i++) {
-        out5[5 * i + 0] = h[i].count;
-        out5[5 * i + 1] = h[i].ha;
-        out5[5 * i + 2] = h[i].hb;
-        out5[5 * i + 3] = h[i].tasks;
-        out5[5 * i + 4] = h[i].alg;
-    }
     return KP_OK;
 }
 
@@ -2033,7 +2226,7 @@ int Engine::stage(const std::vector<JobIn> &jv, std::vector<JobStats> &st,
     size_t S = 0;
     uint32_t lcap = 1, d1cap = 1, stride = 1;
     for (const JobIn &j : jv) {
-        S += size_t(std::max<int64_t>(0, j.pos_end - j.pos_begin));
+        S += j.pos_list ? j.pos_list->size() : size_t(std::max<int64_t>(0, j.pos_end - j.pos_begin));
         const uint32_t pn1 = std::max<uint32_t>(j.g.pn, 1);
         lcap = std::max(lcap, std::min<uint32_t>(pn1, std::max<uint32_t>(2048, 2 * j.g.maxdeg + 1024)));
         d1cap = std::max(d1cap, j.g.d1max);
@@ -2045,12 +2238,17 @@ int Engine::stage(const std::vector<JobIn> &jv, std::vector<JobStats> &st,
     slot_i.reserve(S);
     slot_id.reserve(S);
     slot_p.reserve(S);
-    for (size_t i = 0; i < jv.size(); i++)
-        for (int64_t pos = jv[i].pos_begin; pos < jv[i].pos_end; pos++) {
+    for (size_t i = 0; i < jv.size(); i++) {
+        auto add = [&](uint32_t pos) {
             slot_i.push_back(uint32_t(i));
             slot_id.push_back(jv[i].id);
-            slot_p.push_back(uint32_t(pos));
-        }
+            slot_p.push_back(pos);
+        };
+        if (jv[i].pos_list)
+            for (uint32_t pos : *jv[i].pos_list) add(pos);
+        else
+            for (int64_t pos = jv[i].pos_begin; pos < jv[i].pos_end; pos++) add(uint32_t(pos));
+    }
     const size_t per_slot = size_t(NCAP + lcap) * 4;
     const size_t chunk = std::max<size_t>(1, std::min<size_t>({S, 65536, (size_t(2) << 30) / per_slot}));
     uint32_t nwaves = uint32_t(std::min<size_t>(chunk, size_t(cus) * 16));
@@ -2221,7 +2419,7 @@ int Engine::stage(const std::vector<JobIn> &jv, std::vector<JobStats> &st,
 namespace {
 
 template <int W, int LW>
-int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
+int launch_class(Engine &E, int c, const SearchOpts &o)
 {
     ClassState &C = E.cls[c];
     constexpr int FW = FrameLayout<W>::FW;
@@ -2237,9 +2435,9 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     int nb = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, dyn));
     if (nb < 1) nb = 1;
-    // Untuned, when the epoch's largest class is a wide one (two or more
-    // words, so at most two blocks per CU and VGPRs to spare), each other
-    // class gets one block per CU and runs beside it instead of after it.
+    // Every class with work gets its grid; classes launch together on their
+    // own streams, and idle waves of a class that has nothing left to start
+    // retire, so that blocks of the other classes take their slots.
     // Untuned, a class with fewer pending roots and frames than the waves of
     // a full grid runs at most four blocks per CU (one-word class) or two
     // (wide classes): its few heavy-tailed roots leave most waves polling and
@@ -2249,7 +2447,6 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     // seven).  Batches and served epochs with many roots keep the full grid.
     const uint64_t pending = uint64_t(C.htodo.size() - C.todo_done) + C.nseed + C.spill_n;
     if (T.blocks_per_cu > 0) nb = std::min(nb, T.blocks_per_cu);
-    else if (secondary) nb = 1;
     else if (pending < uint64_t(nb) * uint64_t(E.cus) * 4u) nb = std::min(nb, W >= 2 ? 2 : 4);
     const uint32_t nwg = uint32_t(nb) * uint32_t(E.cus);
     SearchArgs sa{};
@@ -2259,6 +2456,8 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     sa.back_cap = back;
     sa.donate_period = T.donate_period ? T.donate_period : 256u;
     sa.donate_target = std::max<uint32_t>(16, sa.nwaves / 4);
+    sa.retire_margin = std::max<uint32_t>(32, sa.nwaves / 16);
+    sa.retire_ticks = E.ticks_per_ms * 10;
     const size_t nslots = size_t(sa.qcap) * NSUBQ;
     int rc;
     if ((rc = C.q.ensure(nslots * FW * 8)) || (rc = C.qseq.ensure(nslots * 8)) || (rc = C.sched.ensure(sizeof(Sched))) ||
@@ -2322,16 +2521,16 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     return KP_OK;
 }
 
-int launch_class_w(Engine &E, int c, const SearchOpts &o, bool secondary = false)
+int launch_class_w(Engine &E, int c, const SearchOpts &o)
 {
     switch (kClassW[c]) {
-    case 1: return launch_class<1, 2>(E, c, o, secondary);
-    case 2: return launch_class<2, 2>(E, c, o, secondary);
-    case 3: return launch_class<3, 2>(E, c, o, secondary);
-    case 4: return launch_class<4, 1>(E, c, o, secondary);
-    case 6: return launch_class<6, 0>(E, c, o, secondary);
-    case 8: return launch_class<8, 0>(E, c, o, secondary);
-    case 16: return launch_class<16, 0>(E, c, o, secondary);
+    case 1: return launch_class<1, 2>(E, c, o);
+    case 2: return launch_class<2, 2>(E, c, o);
+    case 3: return launch_class<3, 2>(E, c, o);
+    case 4: return launch_class<4, 1>(E, c, o);
+    case 6: return launch_class<6, 0>(E, c, o);
+    case 8: return launch_class<8, 0>(E, c, o);
+    case 16: return launch_class<16, 0>(E, c, o);
     }
     return KP_EINVAL;
 }
@@ -2409,6 +2608,176 @@ int carry_frames(Engine &E, ClassState &C, size_t fbytes, uint32_t D, uint32_t n
 
 } // namespace
 
+int fill_words(hipStream_t st, void *p, uint64_t words, uint64_t v)
+{
+    if (!words) return KP_OK;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(uint32_t((words + 255) / 256)), dim3(256), 0, st, static_cast<uint64_t *>(p),
+                       uint32_t(words), v);
+    HIPCHK(hipGetLastError());
+    return KP_OK;
+}
+
+// Stack of a per-root wave at width W: the DFS depth is at most n + 2 frames
+// (every child shrinks C2 or C), the newest in the LDS ring.
+void root_stack(int W, uint32_t &ring, uint32_t &back)
+{
+    const uint32_t FW = uint32_t(FrameWords(W));
+    const uint32_t need = 64u * uint32_t(W) + 3;
+    ring = std::min(need, std::max<uint32_t>(16, 6144 / (FW * 8)));
+    back = need - ring + 1;
+}
+
+template <int W, int LW>
+int launch_roots(hipStream_t st, const RootKernArgs &ra, uint32_t nwaves)
+{
+    constexpr int FW = FrameLayout<W>::FW;
+    const size_t dyn = size_t(ra.s.ring_cap) * FW * 8 * WPB;
+    hipLaunchKernelGGL((root_kernel<W, LW>), dim3((nwaves + WPB - 1) / WPB), dim3(256), dyn, st, ra);
+    HIPCHK(hipGetLastError());
+    return KP_OK;
+}
+
+// Per-root pass at width W (1, 2, 3, 4 words)
+int launch_roots_w(int W, hipStream_t st, const RootKernArgs &ra, uint32_t nwaves)
+{
+    switch (W) {
+    case 1: return launch_roots<1, 2>(st, ra, nwaves);
+    case 2: return launch_roots<2, 2>(st, ra, nwaves);
+    case 3: return launch_roots<3, 2>(st, ra, nwaves);
+    case 4: return launch_roots<4, 1>(st, ra, nwaves);
+    }
+    return KP_EINVAL;
+}
+
+// root positions [0, pn - q + 2) (src/device_funcs.cu:44-45), optionally windowed
+void root_window(uint32_t pn, const kp_params &P, int64_t &b, int64_t &e)
+{
+    e = std::max<int64_t>(0, int64_t(pn) - P.q + 2);
+    b = 0;
+    if (P.root_end > 0 || P.root_begin > 0) {
+        b = std::min<int64_t>(P.root_begin, e);
+        if (P.root_end > 0) e = std::min<int64_t>(P.root_end, e);
+        e = std::max(e, b);
+    }
+}
+
+int Engine::run_roots(const std::vector<JobIn> &jv, uint64_t budget, uint64_t limit_ticks,
+                      std::vector<std::vector<uint32_t>> &left)
+{
+    left.assign(jv.size(), {});
+    std::vector<uint32_t> sj, sp, si;   // slot: job id, position, index in jv
+    uint32_t lcap = 1, d1cap = 1, stride = 1;
+    for (size_t i = 0; i < jv.size(); i++) {
+        const JobIn &j = jv[i];
+        for (int64_t pos = j.pos_begin; pos < j.pos_end; pos++) {
+            sj.push_back(j.id);
+            sp.push_back(uint32_t(pos));
+            si.push_back(uint32_t(i));
+        }
+        const uint32_t pn1 = std::max<uint32_t>(j.g.pn, 1);
+        lcap = std::max(lcap, std::min<uint32_t>(pn1, std::max<uint32_t>(2048, 2 * j.g.maxdeg + 1024)));
+        d1cap = std::max(d1cap, j.g.d1max);
+        stride = std::max(stride, pn1);
+    }
+    if (sj.empty()) return KP_OK;
+    const uint32_t w1cap = (d1cap + 63) / 64;
+    const uint32_t nwaves = uint32_t(std::min<size_t>(sj.size(), size_t(cus) * 16));
+    // sized for the widest pass (four words)
+    uint32_t ring4 = 0, back4 = 0;
+    root_stack(4, ring4, back4);
+    const uint64_t arena_words = 3ull * 256 * 4 + uint64_t(lcap) * 4 + 128;
+    int rc;
+    const size_t cnt_bytes = size_t(nwaves) * stride * 4;
+    if ((rc = slot_job.ensure(4 * sj.size())) || (rc = slot_pos.ensure(4 * sj.size())) ||
+        (rc = rp_blk.ensure(size_t(nwaves) * NCAP * 4)) || (rc = rp_lft.ensure(size_t(nwaves) * lcap * 4)) ||
+        (rc = rp_info.ensure(size_t(nwaves) * sizeof(RootInfo))) || (rc = rp_wr.ensure(size_t(nwaves) * 8)) ||
+        (rc = ws_b1c.ensure(size_t(nwaves) * d1cap * 4)) || (rc = ws_a1.ensure(size_t(nwaves) * d1cap * w1cap * 8)) ||
+        (rc = rp_arena.ensure(size_t(nwaves) * arena_words * 8)) || (rc = rp_desc.ensure(size_t(nwaves) * sizeof(RootDesc))) ||
+        (rc = rp_stack.ensure(size_t(nwaves) * back4 * FrameWords(4) * 8)) || (rc = rp_def.ensure(4 * (sj.size() + 64))) ||
+        (rc = rp_sched.ensure(sizeof(Sched))))
+        return rc;
+    {
+        void *before = ws_cnt.p;
+        if ((rc = ws_cnt.ensure(cnt_bytes))) return rc;
+        if (ws_cnt.p != before || cnt_bytes > ws_cnt_zeroed) {
+            HIPCHK(hipMemsetAsync(ws_cnt.p, 0, ws_cnt.bytes, stream));
+            ws_cnt_zeroed = ws_cnt.bytes;
+        }
+    }
+    HIPCHK(hipMemsetAsync(rp_sched.p, 0, sizeof(Sched), stream));
+    RootKernArgs ra{};
+    ra.d.jobs = jobs.as<JobDev>();
+    ra.d.slot_job = slot_job.as<uint32_t>();
+    ra.d.slot_pos = slot_pos.as<uint32_t>();
+    ra.d.blk = rp_blk.as<uint32_t>();
+    ra.d.lft = rp_lft.as<uint32_t>();
+    ra.d.lcap = lcap;
+    ra.d.info = rp_info.as<RootInfo>();
+    ra.d.ws_cnt = ws_cnt.as<uint32_t>();
+    ra.d.ws_b1c = ws_b1c.as<uint32_t>();
+    ra.d.ws_a1 = ws_a1.as<uint64_t>();
+    ra.d.cnt_stride = stride;
+    ra.d.d1cap = d1cap;
+    ra.d.w1cap = w1cap;
+    ra.b.jobs = jobs.as<JobDev>();
+    ra.b.blk = rp_blk.as<uint32_t>();
+    ra.b.lft = rp_lft.as<uint32_t>();
+    ra.b.lcap = lcap;
+    ra.b.cnt_stride = stride;
+    ra.b.wr_extra = rp_wr.as<uint64_t>();
+    ra.arena = rp_arena.as<uint64_t>();
+    ra.arena_words = arena_words;
+    ra.desc = rp_desc.as<RootDesc>();
+    ra.deferred = rp_def.as<uint32_t>() + 1;
+    ra.ndeferred = rp_def.as<uint32_t>();
+    ra.budget = budget;
+    ra.s.desc = ra.desc;
+    ra.s.acc = acc.as<JobAcc>();
+    ra.s.sched = rp_sched.as<Sched>();
+    ra.s.gstack = rp_stack.as<uint64_t>();
+    ra.s.limit_ticks = limit_ticks;
+    // With a budget only the one-word pass runs and every root it leaves goes
+    // to the pool; without one, the roots wider than W words go on to the
+    // next width, and those wider than four words to the pool.
+    for (int W : {1, 2, 3, 4}) {
+        if (sj.empty()) break;
+        HIPCHK(hipMemcpyAsync(slot_job.p, sj.data(), 4 * sj.size(), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(slot_pos.p, sp.data(), 4 * sj.size(), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemsetAsync(rp_def.p, 0, 4, stream));
+        const uint32_t nw = uint32_t(std::min<size_t>(nwaves, sj.size()));
+        ra.d.nslots = uint32_t(sj.size());
+        ra.d.nwaves = nw;
+        ra.s.nwaves = nw;
+        root_stack(W, ra.s.ring_cap, ra.s.back_cap);
+        if ((rc = launch_roots_w(W, stream, ra, nw))) return rc;
+        uint32_t nd = 0;
+        Sched hs{};
+        HIPCHK(hipMemcpyAsync(&nd, rp_def.p, 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(&hs, rp_sched.p, sizeof(Sched), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (hs.error) {
+            set_error("per-root kernel (W=%d) reported error %d", W, hs.error);
+            return hs.error;
+        }
+        std::vector<uint32_t> def(nd);
+        if (nd) HIPCHK(hipMemcpy(def.data(), rp_def.as<uint32_t>() + 1, 4 * nd, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> nj, np, ni;
+        for (uint32_t s : def) {
+            if (s >= sj.size()) continue;
+            nj.push_back(sj[s]);
+            np.push_back(sp[s]);
+            ni.push_back(si[s]);
+        }
+        sj.swap(nj);
+        sp.swap(np);
+        si.swap(ni);
+        if (budget) break;
+    }
+    for (size_t t = 0; t < sj.size(); t++) left[si[t]].push_back(sp[t]);
+    for (auto &v : left) std::sort(v.begin(), v.end());
+    return KP_OK;
+}
+
 int Engine::launch_epoch(const SearchOpts &o)
 {
     // the class streams start after everything queued on the main stream
@@ -2437,7 +2806,7 @@ int Engine::launch_epoch(const SearchOpts &o)
     for (size_t oi = 0; oi < order.size() && !rc; oi++) {
         const int c = order[oi].second;
         if (hipStreamWaitEvent(cls[c].stream, ev, 0) != hipSuccess) rc = KP_EHIP;
-        else rc = launch_class_w(*this, c, o, oi > 0 && order[0].second > 0);
+        else rc = launch_class_w(*this, c, o);
         if (!rc && std::getenv("KP_SERIAL_CLASSES") && hipStreamSynchronize(cls[c].stream) != hipSuccess) rc = KP_EHIP;
     }
     (void)hipEventDestroy(ev);
@@ -2540,12 +2909,12 @@ int Engine::finish_epoch(const SearchOpts &o, EpochStats &es)
         if (std::getenv("KP_STATS"))
             std::fprintf(stderr,
                          "[kp] search W=%d todo=%u seeds=%u waves=%u (%d blocks/CU) ring=%u back=%u frames=%llu "
-                         "donated=%llu pops=%llu spills=%llu busy=%.1f%% wave-s=%.1f pause=%u dumped=%u\n",
+                         "donated=%llu pops=%llu spills=%llu busy=%.1f%% wave-s=%.1f pause=%u dumped=%u retired=%u\n",
                          kClassW[c], C.launch_todo, C.nseed, C.nwaves, C.blocks_per_cu, C.ring, C.back,
                          (unsigned long long)hs.frames, (unsigned long long)hs.donated, (unsigned long long)hs.pops,
                          (unsigned long long)hs.spills,
                          100.0 * double(hs.busy_ticks) / double(std::max<uint64_t>(1, hs.busy_ticks + hs.idle_ticks)),
-                         double(hs.busy_ticks + hs.idle_ticks) / 1e8, hs.pause, hs.dump_used);
+                         double(hs.busy_ticks + hs.idle_ticks) / 1e8, hs.pause, hs.dump_used, hs.retired);
         if (hs.error) {
             if (!first_err) {
                 set_error("search kernel (class W=%d) reported error %d", kClassW[c], hs.error);
@@ -2633,18 +3002,6 @@ int dev_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out)
 
 namespace {
 
-// root positions [0, pn - q + 2) (src/device_funcs.cu:44-45), optionally windowed
-void root_window(uint32_t pn, const kp_params &P, int64_t &b, int64_t &e)
-{
-    e = std::max<int64_t>(0, int64_t(pn) - P.q + 2);
-    b = 0;
-    if (P.root_end > 0 || P.root_begin > 0) {
-        b = std::min<int64_t>(P.root_begin, e);
-        if (P.root_end > 0) e = std::min<int64_t>(P.root_end, e);
-        e = std::max(e, b);
-    }
-}
-
 SearchOpts search_opts(const Engine &E, const kp_params &P, int device, int &rc)
 {
     SearchOpts o;
@@ -2702,31 +3059,47 @@ int decode_lists(const std::vector<uint32_t> &listed, const std::vector<uint32_t
     return KP_OK;
 }
 
-// Runs the jobs already in E's job table (ids 0..n-1) to completion: stage,
-// then search epochs until no class holds work.
+// Runs the jobs already in E's job table (ids 0..n-1) to completion: the
+// per-root pass when mono_budget > 0 (small graphs), then staging of the roots
+// left and search epochs until no class holds work.  search_ms covers the
+// per-root pass and the epochs, decompose_ms the staging.
 int run_pool(Engine &E, const std::vector<JobIn> &jobs, const kp_params &P0, std::vector<kp_result *> &res,
-             float &decompose_ms, float &search_ms)
+             float &decompose_ms, float &search_ms, uint64_t mono_budget = 0)
 {
     int rc;
     E.reset_tables();
     if ((rc = E.zero_acc(0, uint32_t(jobs.size())))) return rc;
-    hipEvent_t e0, e1, e2;
+    hipEvent_t e0, em, e1, e2;
     HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&em));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventCreate(&e2));
     struct EvGuard {
-        hipEvent_t a, b, c;
-        ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); (void)hipEventDestroy(c); }
-    } eg{e0, e1, e2};
-    HIPCHK(hipEventRecord(e0, E.stream));
-    std::vector<JobStats> st;
-    // plain and batched runs: one arena segment per staging chunk, reused across runs
-    auto alloc = [&E](size_t, uint64_t words) -> uint64_t * { return E.arena_alloc(words); };
-    if ((rc = E.stage(jobs, st, alloc))) return rc;
-    HIPCHK(hipEventRecord(e1, E.stream));
+        hipEvent_t a, b, c, d;
+        ~EvGuard()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            (void)hipEventDestroy(c);
+            (void)hipEventDestroy(d);
+        }
+    } eg{e0, em, e1, e2};
     int orc = KP_OK;
     SearchOpts o = search_opts(E, P0, E.device, orc);
     if (orc) return orc;
+    HIPCHK(hipEventRecord(e0, E.stream));
+    std::vector<std::vector<uint32_t>> left;
+    std::vector<JobIn> jobs2 = jobs;
+    if (mono_budget && !o.want_list) {
+        if ((rc = E.run_roots(jobs, mono_budget, o.limit_ticks, left))) return rc;
+        for (size_t i = 0; i < jobs2.size(); i++) jobs2[i].pos_list = &left[i];
+    }
+    HIPCHK(hipEventRecord(em, E.stream));
+    std::vector<JobStats> st;
+    // plain and batched runs: one arena segment per staging chunk, reused across runs
+    auto alloc = [&E](size_t, uint64_t words) -> uint64_t * { return E.arena_alloc(words); };
+    if ((rc = E.stage(jobs2, st, alloc))) return rc;
+    HIPCHK(hipEventRecord(e1, E.stream));
     if (o.want_list) {
         if ((rc = E.list.ensure(o.list_cap * 4))) return rc;
     }
@@ -2740,22 +3113,26 @@ int run_pool(Engine &E, const std::vector<JobIn> &jobs, const kp_params &P0, std
     }
     HIPCHK(hipEventRecord(e2, E.stream));
     HIPCHK(hipEventSynchronize(e2));
-    HIPCHK(hipEventElapsedTime(&decompose_ms, e0, e1));
+    float mono_ms = 0;
+    HIPCHK(hipEventElapsedTime(&mono_ms, e0, em));
+    HIPCHK(hipEventElapsedTime(&decompose_ms, em, e1));
     HIPCHK(hipEventElapsedTime(&search_ms, e1, e2));
-    std::vector<uint64_t> a5;
-    if ((rc = E.read_acc(0, uint32_t(jobs.size()), a5))) return rc;
+    search_ms += mono_ms;
+    std::vector<JobAcc> acc;
+    if ((rc = E.read_acc(0, uint32_t(jobs.size()), acc))) return rc;
     for (size_t i = 0; i < jobs.size(); i++) {
         kp_result *r = res[i];
-        r->count = a5[5 * i];
-        r->hash_a = a5[5 * i + 1];
-        r->hash_b = a5[5 * i + 2];
-        r->tasks = a5[5 * i + 3];
-        r->search_alg_bytes = a5[5 * i + 4];
-        r->alg_bytes = st[i].alg_bytes + r->search_alg_bytes;
-        r->adj_bytes = st[i].adj_bytes;
-        r->valid_roots = st[i].valid_roots;
-        r->max_n = st[i].max_n;
-        r->max_l = st[i].max_l;
+        const JobAcc &A = acc[i];
+        r->count = A.count;
+        r->hash_a = A.ha;
+        r->hash_b = A.hb;
+        r->tasks = A.tasks;
+        r->search_alg_bytes = A.alg;
+        r->alg_bytes = st[i].alg_bytes + A.dalg + A.alg;
+        r->adj_bytes = st[i].adj_bytes + A.adj;
+        r->valid_roots = st[i].valid_roots + A.roots;
+        r->max_n = std::max(st[i].max_n, A.max_n);
+        r->max_l = std::max(st[i].max_l, A.max_l);
         r->pn = jobs[i].g.pn;
         r->pm = jobs[i].g.pm;
         r->spilled = E.total_spilled - spilled0;
@@ -2791,7 +3168,10 @@ int dev_run(kp_prepared *h, kp_result *res)
     root_window(h->gv.pn, h->p, j.pos_begin, j.pos_end);
     std::vector<kp_result *> rv{res};
     float dms = 0, sms = 0;
-    if ((rc = run_pool(h->E, {j}, h->p, rv, dms, sms))) return rc;
+    // Small graphs take the per-root pass first: one launch walks the roots
+    // whose subtrees stay under the frame budget; the rest go to the pool.
+    const uint64_t mono = (h->gv.pn <= kMonoMaxPn && h->p.pause_every == 0) ? kMonoBudget : 0;
+    if ((rc = run_pool(h->E, {j}, h->p, rv, dms, sms, mono))) return rc;
     res->preprocess_ms = h->preprocess_ms;
     res->upload_ms = h->upload_ms;
     res->pn_first = h->prep->pn_first;

@@ -66,6 +66,7 @@ struct JobIn {
     GraphView g;
     int k, q;
     int64_t pos_begin, pos_end;   // root positions, already clipped to [0, pn - q + 2)
+    const std::vector<uint32_t> *pos_list = nullptr;   // if set: exactly these positions instead
 };
 
 struct JobStats {
@@ -75,6 +76,7 @@ struct JobStats {
 };
 
 struct SearchOpts {
+    uint64_t mono_budget = 0;     // > 0: a per-root pass first, roots over this many frames go to the pool
     bool want_list = false;
     uint64_t list_cap = 0;        // words
     uint64_t limit_ticks = 0;
@@ -112,6 +114,7 @@ struct EpochStats {
 
 struct Engine {
     int device = 0, cus = 0;
+    uint64_t ticks_per_ms = 100000;    // device wall clock (s_memrealtime) rate
     hipStream_t stream = nullptr, side = nullptr;
     DevBuf jobs, acc;                  // JobDev table, JobAcc per job
     std::vector<kpd::JobDev> hjobs;
@@ -139,6 +142,13 @@ struct Engine {
     // returns zeroed device memory for `words` 64-bit words of job index i.
     int stage(const std::vector<JobIn> &jobs, std::vector<JobStats> &st,
               const std::function<uint64_t *(size_t i, uint64_t words)> &alloc);
+    // Monolithic per-root pass over the roots of `jobs` (root_kernel, one wave
+    // per root, widths 1-4 words in turn).  Roots it completes are added to
+    // the jobs' accumulators; the positions it leaves (blocks wider than four
+    // words, subtrees over `budget` frames) are returned per job in `left`.
+    int run_roots(const std::vector<JobIn> &jobs, uint64_t budget, uint64_t limit_ticks,
+                  std::vector<std::vector<uint32_t>> &left);
+    DevBuf rp_blk, rp_lft, rp_info, rp_wr, rp_arena, rp_desc, rp_stack, rp_def, rp_sched;
     // One epoch: launch every class with work; finish waits and carries pauses.
     int launch_epoch(const SearchOpts &o);
     int finish_epoch(const SearchOpts &o, EpochStats &es);
@@ -148,7 +158,7 @@ struct Engine {
     // segment kept across runs (reset_tables frees them all for reuse)
     uint64_t *arena_alloc(uint64_t words);
     void release_arena();              // gives the segments back to the device
-    int read_acc(uint32_t first, uint32_t count, std::vector<uint64_t> &out5);   // 5 words per job
+    int read_acc(uint32_t first, uint32_t count, std::vector<kpd::JobAcc> &out);
     // pending frames (device seeds and host spill) per job index; per_job must have room
     int count_pending(std::vector<uint64_t> &per_job);
     // allocates the pinned, device-mapped pause word that SearchOpts::host_pause polls
@@ -160,5 +170,14 @@ struct Engine {
 };
 
 int dev_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r);
+
+// root positions [0, pn - q + 2) of a run, clipped to the window of *p
+void root_window(uint32_t pn, const kp_params &p, int64_t &b, int64_t &e);
+// LDS ring and HBM backing frames per wave of the per-root kernel at width W
+void root_stack(int W, uint32_t &ring, uint32_t &back);
+// launches the per-root kernel of width W (1, 2, 3 or 4 words) over ra's slots
+int launch_roots_w(int W, hipStream_t st, const kpd::RootKernArgs &ra, uint32_t nwaves);
+// fills `words` 64-bit words at p with v (a kernel launch, no runtime allocation)
+int fill_words(hipStream_t st, void *p, uint64_t words, uint64_t v);
 
 } // namespace kp

@@ -21,11 +21,16 @@ struct JobDev {
     uint32_t pad;
 };
 
-// Per-job result accumulators (sums mod 2^64), one 64-byte line each.
+// Per-job result accumulators (sums mod 2^64).  The second line holds what
+// the monolithic per-root kernel adds for the roots it completes (the pooled
+// path counts its roots on the host while staging).
 struct JobAcc {
-    unsigned long long count, ha, hb, tasks, alg, pad[3];
+    unsigned long long count, ha, hb, tasks, alg, pad0[3];
+    unsigned long long roots, adj, dalg;   // valid roots, 4 W_r, 4 W_r + ceil(n^2/8) + n^2
+    unsigned int max_n, max_l;
+    unsigned long long pad1[4];
 };
-static_assert(sizeof(JobAcc) == 64, "JobAcc is one line");
+static_assert(sizeof(JobAcc) == 128, "JobAcc is two lines");
 
 struct RootDesc {          // one valid root; frames name roots by their index in the class table
     uint64_t addr;         // device address of the root's subgraph
@@ -45,12 +50,95 @@ struct Sched {             // persistent-kernel scheduler state (one per launch)
     uint32_t exited;
     uint32_t dump_used;    // frames written to the dump array
     int32_t error;
-    uint32_t pad0;
+    uint32_t retired;      // waves that exited early as surplus
     uint64_t epoch_frames; // frames processed, in steps of the check interval
     uint64_t list_unused;
     uint64_t donated;
     uint64_t frames, spills, pops;
     uint64_t busy_ticks, idle_ticks;   // wall-clock ticks summed over waves
+};
+
+
+// Words of a task frame of a W-word size class: meta, P, C, X, C2, rounded
+// to an even count so that frames stay 16-byte aligned.
+constexpr int FrameWords(int W) { return (4 * W + 2) & ~1; }
+
+// ---- kernel arguments (built on the host, passed by value) ----
+
+// One shard of the frame queue: a bounded MPMC ring with sequence-numbered
+// slots.  Shards sit on separate 128-byte lines so that waves polling and
+// claiming different shards do not contend on one address.
+struct SubQ {
+    uint32_t count;        // published frames
+    uint32_t size;         // reserved slots
+    uint64_t head, tail;   // tickets
+    uint32_t pad[26];
+};
+static_assert(sizeof(SubQ) == 128, "SubQ is one 128-byte line");
+constexpr uint32_t NSUBQ = 32;
+
+struct DecompArgs {
+    const JobDev *jobs;
+    const uint32_t *slot_job, *slot_pos;   // per slot: job index and root position
+    uint32_t nslots;
+    uint32_t *blk, *lft;
+    uint32_t lcap;
+    RootInfo *info;
+    uint32_t *ws_cnt, *ws_b1c;
+    uint64_t *ws_a1;
+    uint32_t cnt_stride;                   // counter words per wave (largest pn)
+    uint32_t d1cap, w1cap, nwaves;
+};
+
+struct BuildArgs {
+    const JobDev *jobs;
+    const uint32_t *blk, *lft;
+    uint32_t lcap;
+    const RootDesc *desc;
+    uint32_t ndesc;
+    uint32_t *ws_cnt;
+    uint32_t cnt_stride, nwaves;
+    uint64_t *wr_extra;   // per desc: sum of degrees over B and L
+};
+
+struct SearchArgs {
+    const RootDesc *desc;      // class root table
+    const uint32_t *todo;      // roots to start, in order (indices into desc)
+    uint32_t ntodo;
+    const uint64_t *seed;      // frames to resume
+    uint32_t nseed;
+    uint64_t *dump;            // frames written on pause
+    uint32_t dump_cap;
+    JobAcc *acc;               // per-job accumulators
+    uint64_t *qframes, *qseq;  // NSUBQ * qcap slots
+    uint32_t qcap;             // slots per shard
+    SubQ *subq;
+    Sched *sched;
+    uint32_t *list;
+    uint64_t list_cap;
+    uint64_t *list_used;       // shared by all size-class launches
+    uint32_t nwaves;
+    uint32_t ring_cap;         // frames per wave in the LDS ring
+    uint32_t back_cap;         // frames per wave in the HBM backing stack
+    uint64_t *gstack;          // nwaves * back_cap frames
+    uint32_t donate_period, donate_target;
+    uint64_t limit_ticks;      // 0 = no limit; else wall-clock ticks per launch
+    const uint32_t *host_pause;   // mapped host word (nonzero = pause) or null
+    uint64_t pause_frames;     // 0 = none; else pause once this many frames were processed
+    unsigned long long *t0;    // shared start tick of the epoch (0 until the first wave of any class starts)
+    uint32_t retire_margin;    // hungry waves kept beyond the busy ones once nothing is left to start
+    uint64_t retire_ticks;     // wall-clock ticks a wave stays hungry before it may retire
+};
+
+struct RootKernArgs {
+    DecompArgs d;              // jobs, slot lists, workspaces; blk/lft/info hold one entry per wave
+    BuildArgs b;               // jobs, blk/lft, lcap; wr_extra one entry per wave
+    uint64_t *arena;           // arena_words per wave
+    uint64_t arena_words;
+    RootDesc *desc;            // one entry per wave (the search reads it as its root table)
+    uint32_t *deferred, *ndeferred;
+    uint64_t budget;           // frames per root before it is deferred (0 = none)
+    SearchArgs s;
 };
 
 } // namespace kpd

@@ -176,17 +176,17 @@ void free_regions(kp_server *s, Live &L)
 int complete(kp_server *s, uint32_t slot, double now)
 {
     Live &L = s->live[slot];
-    std::vector<uint64_t> a5;
-    int rc = s->E.read_acc(slot, 1, a5);
+    std::vector<kpd::JobAcc> acc;
+    int rc = s->E.read_acc(slot, 1, acc);
     if (rc) return rc;
     kp_result r;
     std::memset(&r, 0, sizeof r);
-    r.count = a5[0];
-    r.hash_a = a5[1];
-    r.hash_b = a5[2];
-    r.tasks = a5[3];
-    r.search_alg_bytes = a5[4];
-    r.alg_bytes = L.st.alg_bytes + a5[4];
+    r.count = acc[0].count;
+    r.hash_a = acc[0].ha;
+    r.hash_b = acc[0].hb;
+    r.tasks = acc[0].tasks;
+    r.search_alg_bytes = acc[0].alg;
+    r.alg_bytes = L.st.alg_bytes + acc[0].alg;
     r.adj_bytes = L.st.adj_bytes;
     r.valid_roots = L.st.valid_roots;
     r.max_n = L.st.max_n;

@@ -1,12 +1,15 @@
 """GPU tests of the shared pool: many graphs in one set of launches
 (kp_enumerate_batch) and pause / resume with pinned-host spill.  Every
 result must equal the plain one-graph run and the oracle."""
+import json
+import os
 import sys
 
 import pytest
 
 import kplex
 import kpo
+from conftest import GOLDEN
 from kp_testutil import csr, disjoint_cliques, multipartite, random_graph
 
 pytestmark = pytest.mark.gpu
@@ -88,19 +91,28 @@ def test_batch_many_chung_lu_graphs():
     assert tot > 0
 
 
+def batch_gold():
+    with open(os.path.join(GOLDEN, "oracle_batch.json")) as f:
+        return json.load(f)["items"]
+
+
 def test_batch_with_more_roots_than_resident_waves():
     # The first 48 graphs of the many-graph workload: thousands of roots over
     # several size classes whose concurrent grids do not all fit on the chip
     # at once.  A wave must count as busy only once it runs, or idle resident
     # waves of one class wait on waves of another that need their slots.
+    # Every graph's result must equal the oracle's (tests/golden/oracle_batch.json).
+    gold = batch_gold()
     specs = kplex.batch_graph_specs()[:48]
     graphs = [kplex.graph_from_spec(s) for s in specs]
     say(f"batch of the first {len(graphs)} many-graph members")
     rs = kplex.enumerate_batch(graphs, [dict(k=s["k"], q=s["q"], time_limit_s=LIMIT) for s in specs])
+    for i, r in enumerate(rs):
+        same(r, gold[str(i)])
     assert sum(r["count"] for r in rs) == 1584447134
     for i in (0, 11, 47):
         say(f"  plain member {i}")
-        same(rs[i], kplex.enumerate_plexes(graphs[i], specs[i]["k"], specs[i]["q"], time_limit_s=LIMIT))
+        same(kplex.enumerate_plexes(graphs[i], specs[i]["k"], specs[i]["q"], time_limit_s=LIMIT), gold[str(i)])
 
 
 def mid_graph():

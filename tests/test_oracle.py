@@ -132,3 +132,21 @@ def test_k_search_budget_above_256():
     # k = 257: the k-search budget k-1 = 256 must not wrap
     r = kpo.enumerate_plexes(ograph(clique_plus_one(600)), 257, 514, listing=True, threads=4)
     assert r["count"] == 1 and r["plexes"] == [tuple(range(601))]
+
+
+def test_batch_fixtures_hold_for_the_quick_members():
+    # tests/golden/oracle_batch.json (the oracle on the first 48 many-graph
+    # members) against a fresh oracle run on the members that take well under
+    # a second, so the committed file cannot drift from the oracle
+    import kplex
+    with open(os.path.join(GOLDEN, "oracle_batch.json")) as f:
+        gold = json.load(f)["items"]
+    specs = kplex.batch_graph_specs()[:48]
+    quick = sorted(range(48), key=lambda i: gold[str(i)]["oracle_s"])[:5]
+    for i in quick:
+        g = kplex.graph_from_spec(specs[i])
+        deg, nbr = g.csr()
+        r = kpo.enumerate_plexes(kpo.OGraph.from_csr(deg, nbr), specs[i]["k"], specs[i]["q"], threads=4)
+        for x in ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l"):
+            assert r[x] == gold[str(i)][x], (i, x)
+    assert sum(v["count"] for v in gold.values()) == 1584447134
