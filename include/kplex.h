@@ -126,6 +126,20 @@ typedef struct kp_result {
 int  kp_enumerate(const kp_graph *g, const kp_params *p, kp_result *r);
 void kp_result_free(kp_result *r);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same enumeration as kp_enumerate, same arguments and
+ * results, with no device or pinned-host allocation.  Every device buffer is
+ * carved out of one static device array of KP_NOALLOC_SCRATCH_BYTES that each
+ * GPU holds from the library's first use on it until the process ends; the
+ * enumeration runs as the monolithic per-root kernels (one wave per root) on
+ * the null stream.  Limits: the preprocessed graph and the per-wave buffers
+ * must fit the array (else KP_ENOMEM, with the bytes needed in
+ * kp_last_error), root blocks of at most 256 vertices (else KP_ERANGE); no
+ * plex lists and no pauses (KP_EINVAL).  Calls that overlap on one GPU race
+ * on the shared array: serialize them. */
+#define KP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int  kp_enumerate_noalloc(const kp_graph *g, const kp_params *p, kp_result *r);
 
 /* Host preprocessing only (no device needed): the (q-k)-core after the
  * optional truss (inc/host_funcs.h:749-823), its new->original id map and the

@@ -102,6 +102,19 @@ extern "C" int kp_enumerate(const kp_graph *g, const kp_params *p, kp_result *r)
     return dev_enumerate(g, p, r);
 }
 
+extern "C" int kp_enumerate_noalloc(const kp_graph *g, const kp_params *p, kp_result *r)
+{
+    if (!r) return KP_EINVAL;
+    std::memset(r, 0, sizeof(*r));
+    int rc = check_params(g, p);
+    if (rc) return rc;
+    if ((p->flags & KP_FLAG_LIST) || p->pause_every) {
+        set_error("kp_enumerate_noalloc returns counts and hashes without pauses (no KP_FLAG_LIST, no pause_every)");
+        return KP_EINVAL;
+    }
+    return dev_enumerate_noalloc(g, p, r);
+}
+
 extern "C" int kp_enumerate_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r)
 {
     if (n && (!g || !p || !r)) return KP_EINVAL;

@@ -9,6 +9,7 @@ namespace kp {
 int  dev_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out);
 int  dev_run(kp_prepared *h, kp_result *r);
 int  dev_enumerate(const kp_graph *g, const kp_params *p, kp_result *r);
+int  dev_enumerate_noalloc(const kp_graph *g, const kp_params *p, kp_result *r);
 int  dev_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result *r);
 int  dev_autotune(kp_prepared *h, kp_tune *out);
 void dev_set_tune(const kp_tune *t);

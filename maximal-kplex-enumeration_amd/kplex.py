@@ -72,6 +72,7 @@ SIGNATURES = {
     "kp_graph_validate": (c_int, [POINTER(KpGraph)]),
     "kp_graph_free": (None, [POINTER(KpGraph)]),
     "kp_enumerate": (c_int, [POINTER(KpGraph), POINTER(KpParams), POINTER(KpResult)]),
+    "kp_enumerate_noalloc": (c_int, [POINTER(KpGraph), POINTER(KpParams), POINTER(KpResult)]),
     "kp_result_free": (None, [POINTER(KpResult)]),
     "kp_prepare": (c_int, [POINTER(KpGraph), POINTER(KpParams), POINTER(c_void_p)]),
     "kp_run": (c_int, [c_void_p, POINTER(KpResult)]),
