@@ -128,7 +128,11 @@ def batched_vs_plain(ngraphs, time_limit):
     log(f"[bench] batched: {len(graphs)} graphs, n {min(s['n'] for s in specs)}..{max(s['n'] for s in specs)}")
     kplex.enumerate_plexes(graphs[0], *params[0], time_limit_s=time_limit)   # warm the library
     t0 = time.perf_counter()
-    plain = [kplex.enumerate_plexes(g, k, q, time_limit_s=time_limit) for g, (k, q) in zip(graphs, params)]
+    plain = []
+    for i, (g, (k, q)) in enumerate(zip(graphs, params)):
+        plain.append(kplex.enumerate_plexes(g, k, q, time_limit_s=time_limit))
+        if (i + 1) % 256 == 0:
+            log(f"[bench] batched: plain {i + 1}/{len(graphs)} graphs in {time.perf_counter() - t0:.1f} s")
     plain_s = time.perf_counter() - t0
     log(f"[bench] batched: plain {plain_s:.2f} s")
     t0 = time.perf_counter()
@@ -289,6 +293,42 @@ def small_line(reps=7):
             "gpu_vs_cpu": c_med / g_med, "identical": True}
 
 
+def full_graph_line(time_limit):
+    """The Chung-Lu 75k workload in full: every root position, enumerated as
+    consecutive windows (windows partition the result, so counts, task counts
+    and hashes add up), each window one kp_run that prints its line when it
+    ends; the last, heaviest positions go in narrower windows.  The sum is the
+    whole graph's exact count."""
+    import kplex
+    desc, fn, _ = WORKLOADS["cl75k-full"]
+    g, k, q, thres, tr = getattr(kplex, fn)()
+    end = len(kplex.preprocess(g, k, q, tr)[0]) - q + 2
+    cuts = [0, 9000] + list(range(9500, min(end, 16000), 500)) + list(range(16000, end, 50)) + [end]
+    cuts = sorted(set(c for c in cuts if c <= end))
+    windows = list(zip(cuts[:-1], cuts[1:]))
+    tot = {"count": 0, "tasks": 0, "hash_a": 0, "hash_b": 0, "search_ms": 0.0, "valid_roots": 0}
+    per = []
+    t0 = time.perf_counter()
+    for w in windows:
+        p = kplex.Prepared(g, k, q, thres, tr, time_limit_s=time_limit, roots=w)
+        r = p.run()
+        p.close()
+        for x in ("count", "tasks", "valid_roots"):
+            tot[x] += r[x]
+        tot["hash_a"] = (tot["hash_a"] + r["hash_a"]) % (1 << 64)
+        tot["hash_b"] = (tot["hash_b"] + r["hash_b"]) % (1 << 64)
+        tot["search_ms"] += r["search_ms"] + r["decompose_ms"]
+        per.append({"window": list(w), "count": r["count"], "hash": r["hash"], "tasks": r["tasks"],
+                    "gpu_ms": r["search_ms"] + r["decompose_ms"]})
+        log(f"[bench] full graph window {w}: count={r['count']} tasks={r['tasks']} gpu_ms={per[-1]['gpu_ms']:.0f} "
+            f"(total {tot['count']} after {time.perf_counter() - t0:.1f} s)")
+    wall = time.perf_counter() - t0
+    return {"workload": desc, "roots": end, "windows": len(windows), "count": tot["count"],
+            "hash": f"{tot['hash_b']:016x}{tot['hash_a']:016x}", "tasks": tot["tasks"],
+            "valid_roots": tot["valid_roots"], "gpu_s": tot["search_ms"] / 1e3, "wall_s": wall,
+            "plexes_per_s": tot["count"] / wall, "per_window": per}
+
+
 def copy_calibration(nbytes):
     """One device-to-device copy of `nbytes` (hipMemcpy: a copy kernel that reads
     and writes exactly nbytes).  Run under rocprofv3 --pmc FETCH_SIZE or
@@ -327,11 +367,17 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="time with the built-in launch settings")
     ap.add_argument("--no-spill-line", action="store_true", help="skip the pause/spill line on the 320k workload")
     ap.add_argument("--no-small-line", action="store_true", help="skip the GPU-vs-CPU line on the small planted workload")
+    ap.add_argument("--full-graph", action="store_true",
+                    help="also enumerate the whole Chung-Lu 75k graph (every root; many minutes of GPU time)")
+    ap.add_argument("--full-graph-only", action="store_true", help="only the whole-graph line, printed as JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device (counter calibration for the traffic figure) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.full_graph_only:
+        print(json.dumps({"full_graph": full_graph_line(args.time_limit)}))
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -419,6 +465,8 @@ def main():
         out["batched"] = batched_vs_plain(args.batch_graphs, args.time_limit)
     if args.served_seconds > 0:
         out["served"] = served_vs_plain(args.served_seconds, args.served_rate, args.time_limit)
+    if args.full_graph:
+        out["full_graph"] = full_graph_line(args.time_limit)
     # the other parts of the metric, at the top level
     out["wall_s_at_exact_count"] = out["wall_s_per_step"]
     out["batched_vs_plain"] = out["batched"]["batched_vs_plain"] if "batched" in out else None

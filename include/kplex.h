@@ -157,6 +157,18 @@ typedef struct kp_prepared kp_prepared;
 int  kp_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out);
 int  kp_run(kp_prepared *h, kp_result *r);
 uint64_t kp_prepared_bytes(const kp_prepared *h);
+/* Inspection hook for root-by-root parity: the decomposition and subgraph of
+ * the root at degeneracy position pos of a prepared graph, as the GPU builds
+ * them -- what the reference's decompose, calculateDegrees / fillNeighbors and
+ * buildCommonMtx compute for one root (src/device_funcs.cu:28-460,
+ * :1727-1853).  block (room for KP_MAX_BLOCK) and left (room for left_cap)
+ * receive peeled vertex ids in the canonical order (the left set's 2-hop part
+ * in the order the GPU claimed it: the left set is used as a set); pairs, unless NULL,
+ * receives n*n bytes, pairs[i*n+j] bit 0 = block[i] adjacent to block[j],
+ * bit 1 = pair class >= UNLINK2EQUAL, bit 2 = pair class >= UNLINK2MORE.
+ * Returns 1 for a valid root (n >= q), 0 for an invalid one, < 0 on error. */
+int  kp_root_subgraph(kp_prepared *h, int64_t pos, uint32_t *block, uint32_t *n, uint32_t *hop,
+                      uint32_t *left, uint32_t *l, uint32_t left_cap, uint8_t *pairs, size_t pairs_cap);
 void kp_prepared_free(kp_prepared *h);
 
 /* ---- launch settings per size class (autotuner) --------------------- */

@@ -91,6 +91,13 @@ extern "C" int kp_autotune(kp_prepared *h, kp_tune *out)
 
 extern "C" void kp_set_tune(const kp_tune *t) { dev_set_tune(t); }
 
+extern "C" int kp_root_subgraph(kp_prepared *h, int64_t pos, uint32_t *block, uint32_t *n, uint32_t *hop,
+                                uint32_t *left, uint32_t *l, uint32_t left_cap, uint8_t *pairs, size_t pairs_cap)
+{
+    if (!h || !block || !n || !hop || !left || !l) return KP_EINVAL;
+    return dev_root_subgraph(h, pos, block, n, hop, left, l, left_cap, pairs, pairs_cap);
+}
+
 extern "C" void kp_prepared_free(kp_prepared *h) { dev_free(h); }
 
 extern "C" int kp_enumerate(const kp_graph *g, const kp_params *p, kp_result *r)

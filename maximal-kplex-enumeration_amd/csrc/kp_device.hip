@@ -2419,7 +2419,7 @@ int Engine::stage(const std::vector<JobIn> &jv, std::vector<JobStats> &st,
 namespace {
 
 template <int W, int LW>
-int launch_class(Engine &E, int c, const SearchOpts &o)
+int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
 {
     ClassState &C = E.cls[c];
     constexpr int FW = FrameLayout<W>::FW;
@@ -2435,9 +2435,16 @@ int launch_class(Engine &E, int c, const SearchOpts &o)
     int nb = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, dyn));
     if (nb < 1) nb = 1;
-    // Every class with work gets its grid; classes launch together on their
-    // own streams, and idle waves of a class that has nothing left to start
-    // retire, so that blocks of the other classes take their slots.
+    // Classes launch together on their own streams, and idle waves of a class
+    // that has nothing left to start retire, so that blocks of the other
+    // classes take their slots.  Untuned, when the epoch's primary class (the
+    // one with the most pending roots) is a wide one (two or more words, so at
+    // most two blocks per CU and VGPRs to spare), each other class gets one
+    // block per CU and runs beside it.  Measured: email-euall with the
+    // two-word class primary 150 ms this way against 250 ms with every class
+    // on its full grid; a window of the Chung-Lu 75k graph whose one-word
+    // class holds most frames, 12.6 s with the one-word class primary against
+    // 29 s with a wider class primary (chosen then by the sum of n^2).
     // Untuned, a class with fewer pending roots and frames than the waves of
     // a full grid runs at most four blocks per CU (one-word class) or two
     // (wide classes): its few heavy-tailed roots leave most waves polling and
@@ -2447,6 +2454,7 @@ int launch_class(Engine &E, int c, const SearchOpts &o)
     // seven).  Batches and served epochs with many roots keep the full grid.
     const uint64_t pending = uint64_t(C.htodo.size() - C.todo_done) + C.nseed + C.spill_n;
     if (T.blocks_per_cu > 0) nb = std::min(nb, T.blocks_per_cu);
+    else if (secondary) nb = 1;
     else if (pending < uint64_t(nb) * uint64_t(E.cus) * 4u) nb = std::min(nb, W >= 2 ? 2 : 4);
     const uint32_t nwg = uint32_t(nb) * uint32_t(E.cus);
     SearchArgs sa{};
@@ -2521,16 +2529,16 @@ int launch_class(Engine &E, int c, const SearchOpts &o)
     return KP_OK;
 }
 
-int launch_class_w(Engine &E, int c, const SearchOpts &o)
+int launch_class_w(Engine &E, int c, const SearchOpts &o, bool secondary = false)
 {
     switch (kClassW[c]) {
-    case 1: return launch_class<1, 2>(E, c, o);
-    case 2: return launch_class<2, 2>(E, c, o);
-    case 3: return launch_class<3, 2>(E, c, o);
-    case 4: return launch_class<4, 1>(E, c, o);
-    case 6: return launch_class<6, 0>(E, c, o);
-    case 8: return launch_class<8, 0>(E, c, o);
-    case 16: return launch_class<16, 0>(E, c, o);
+    case 1: return launch_class<1, 2>(E, c, o, secondary);
+    case 2: return launch_class<2, 2>(E, c, o, secondary);
+    case 3: return launch_class<3, 2>(E, c, o, secondary);
+    case 4: return launch_class<4, 1>(E, c, o, secondary);
+    case 6: return launch_class<6, 0>(E, c, o, secondary);
+    case 8: return launch_class<8, 0>(E, c, o, secondary);
+    case 16: return launch_class<16, 0>(E, c, o, secondary);
     }
     return KP_EINVAL;
 }
@@ -2786,27 +2794,23 @@ int Engine::launch_epoch(const SearchOpts &o)
     HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     HIPCHK(hipEventRecord(ev, stream));
     int rc = KP_OK;
-    // Classes with the most pending work (sum of n^2 over roots not yet
-    // started, resumed frames counted at the class's largest n) launch first:
+    // Classes with the most pending roots and resumed frames launch first:
     // each launch fills the device, and later ones start on the CUs that the
-    // earlier ones free as they drain.
+    // earlier ones free.  (The size of a block says little about its work: on
+    // the Chung-Lu 75k graph, roots of 35-60 dense community vertices hold far
+    // more frames than hub roots of 200-380 sparse ones.)
     std::vector<std::pair<double, int>> order;
     for (int c = 0; c < kNumClass; c++) {
         cls[c].launched = false;
         if (!cls[c].has_work()) continue;
         const ClassState &C = cls[c];
-        double w = double(C.nseed + C.spill_n) * double(C.maxn) * double(C.maxn);
-        for (size_t t = C.todo_done; t < C.htodo.size(); t++) {
-            const double n = double(C.hdesc[C.htodo[t]].n);
-            w += n * n;
-        }
-        order.emplace_back(-w, c);
+        order.emplace_back(-double(C.htodo.size() - C.todo_done + C.nseed + C.spill_n), c);
     }
     std::stable_sort(order.begin(), order.end());
     for (size_t oi = 0; oi < order.size() && !rc; oi++) {
         const int c = order[oi].second;
         if (hipStreamWaitEvent(cls[c].stream, ev, 0) != hipSuccess) rc = KP_EHIP;
-        else rc = launch_class_w(*this, c, o);
+        else rc = launch_class_w(*this, c, o, oi > 0 && order[0].second > 0);
         if (!rc && std::getenv("KP_SERIAL_CLASSES") && hipStreamSynchronize(cls[c].stream) != hipSuccess) rc = KP_EHIP;
     }
     (void)hipEventDestroy(ev);
@@ -3248,6 +3252,63 @@ int dev_enumerate(const kp_graph *g, const kp_params *p, kp_result *r)
         C.h = nullptr;
     }
     return rc;
+}
+
+// One root's decomposition and subgraph as the GPU builds them (the staging
+// path: decompose_kernel, then build_kernel of the root's size class), copied
+// back: peeled ids of the block and the left set in canonical order and, per
+// pair of block vertices, adjacency and the two class thresholds the search
+// tests.  Returns 1 for a valid root, 0 for an invalid one.
+int dev_root_subgraph(kp_prepared *h, int64_t pos, uint32_t *block, uint32_t *n, uint32_t *hop, uint32_t *left,
+                      uint32_t *l, uint32_t left_cap, uint8_t *pairs, size_t pairs_cap)
+{
+    HIPCHK(hipSetDevice(h->E.device));
+    Engine &E = h->E;
+    *n = *hop = *l = 0;
+    if (pos < 0 || pos >= int64_t(h->gv.pn)) return KP_EINVAL;
+    int rc;
+    if ((rc = E.reserve_jobs(1))) return rc;
+    E.set_job(0, h->gv, h->p.k, h->p.q);
+    if ((rc = E.sync_jobs())) return rc;
+    E.reset_tables();
+    JobIn j{0, h->gv, h->p.k, h->p.q, pos, pos + 1};
+    std::vector<JobStats> st;
+    auto alloc = [&E](size_t, uint64_t words) -> uint64_t * { return E.arena_alloc(words); };
+    if ((rc = E.stage({j}, st, alloc))) return rc;
+    RootInfo ri{};
+    HIPCHK(hipMemcpy(&ri, E.info.p, sizeof ri, hipMemcpyDeviceToHost));
+    int c = -1;
+    for (int x = 0; x < kNumClass; x++)
+        if (!E.cls[x].hdesc.empty()) c = x;
+    if (ri.status != 1 || c < 0) {
+        E.reset_tables();
+        return 0;
+    }
+    const RootDesc d = E.cls[c].hdesc[0];
+    const uint32_t W = uint32_t(kClassW[c]);
+    if (d.l > left_cap || (pairs && size_t(d.n) * d.n > pairs_cap)) {
+        E.reset_tables();
+        set_error("root at position %lld: block %u / left set %u exceed the caller's buffers", (long long)pos, d.n, d.l);
+        return KP_ERANGE;
+    }
+    std::vector<uint64_t> rows(size_t(3) * d.n * W);
+    HIPCHK(hipMemcpy(block, E.blk.p, 4 * size_t(d.n), hipMemcpyDeviceToHost));
+    if (d.l) HIPCHK(hipMemcpy(left, E.lft.p, 4 * size_t(d.l), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rows.data(), reinterpret_cast<const void *>(d.addr), 8 * rows.size(), hipMemcpyDeviceToHost));
+    if (pairs) {
+        const uint64_t *A = rows.data(), *C2 = A + size_t(d.n) * W, *C4 = C2 + size_t(d.n) * W;
+        for (uint32_t i = 0; i < d.n; i++)
+            for (uint32_t t = 0; t < d.n; t++) {
+                const size_t wd = size_t(i) * W + (t >> 6);
+                const uint64_t b = 1ull << (t & 63);
+                pairs[size_t(i) * d.n + t] = uint8_t(((A[wd] & b) ? 1 : 0) | ((C2[wd] & b) ? 2 : 0) | ((C4[wd] & b) ? 4 : 0));
+            }
+    }
+    *n = d.n;
+    *hop = d.h;
+    *l = d.l;
+    E.reset_tables();
+    return 1;
 }
 
 // Coordinate descent over the launch settings of each size class, timing the

@@ -78,6 +78,8 @@ SIGNATURES = {
     "kp_run": (c_int, [c_void_p, POINTER(KpResult)]),
     "kp_prepared_bytes": (c_uint64, [c_void_p]),
     "kp_prepared_free": (None, [c_void_p]),
+    "kp_root_subgraph": (c_int, [c_void_p, ctypes.c_int64, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
+                                 POINTER(c_uint32), POINTER(c_uint32), c_uint32, POINTER(ctypes.c_uint8), c_size_t]),
     "kp_autotune": (c_int, [c_void_p, POINTER(KpTune)]),
     "kp_set_tune": (None, [POINTER(KpTune)]),
     "kp_enumerate_batch": (c_int, [POINTER(POINTER(KpGraph)), POINTER(KpParams), c_size_t, POINTER(KpResult)]),
@@ -320,6 +322,27 @@ class Prepared:
 
     def bytes(self):
         return self.lib.kp_prepared_bytes(self.h)
+
+    def root_subgraph(self, pos):
+        """kp_root_subgraph: the root at degeneracy position pos as the GPU
+        decomposes and builds it.  Returns None for an invalid root, else a dict
+        with block, hop, left (peeled ids) and pairs (n x n uint8: bit 0 adjacent,
+        bit 1 class >= UNLINK2EQUAL, bit 2 class >= UNLINK2MORE)."""
+        cap = 1 << 16
+        block = np.zeros(1024, dtype=np.uint32)
+        left = np.zeros(cap, dtype=np.uint32)
+        pairs = np.zeros(1024 * 1024, dtype=np.uint8)
+        n, hop, l = c_uint32(), c_uint32(), c_uint32()
+        rc = self.lib.kp_root_subgraph(self.h, pos, block.ctypes.data_as(POINTER(c_uint32)), ctypes.byref(n),
+                                       ctypes.byref(hop), left.ctypes.data_as(POINTER(c_uint32)), ctypes.byref(l), cap,
+                                       pairs.ctypes.data_as(POINTER(ctypes.c_uint8)), pairs.size)
+        if rc < 0:
+            _check(rc, "kp_root_subgraph")
+        if rc == 0:
+            return None
+        nn = n.value
+        return {"block": block[:nn].copy(), "hop": hop.value, "left": left[:l.value].copy(),
+                "pairs": pairs[:nn * nn].reshape(nn, nn).copy()}
 
     def autotune(self):
         """kp_autotune: returns {class words W: settings} for the classes present."""

@@ -158,3 +158,42 @@ def hash_plexes(plexes):
         ha = (ha + py_mix64((sa + len(s) * LA) & M64)) & M64
         hb = (hb + py_mix64((sb + len(s) * LB) & M64)) & M64
     return f"{hb:016x}{ha:016x}"
+
+
+class RootBlocks:
+    """kpo_root_block over one preprocessed graph: the oracle's decomposition
+    and pair-class matrix of single roots."""
+
+    def __init__(self, og, k, q, truss=0):
+        self.lib = load()
+        self.p = KpoPrep()
+        self.k, self.q = k, q
+        rc = self.lib.kpo_preprocess(ctypes.byref(og.g), k, q, truss, ctypes.byref(self.p))
+        if rc:
+            raise RuntimeError(f"kpo_preprocess = {rc}")
+        self.pn = self.p.g.n
+
+    def __del__(self):
+        try:
+            self.lib.kpo_prep_free(ctypes.byref(self.p))
+        except Exception:
+            pass
+
+    def block(self, pos):
+        """None for an invalid root, else block, hop, left (peeled ids) and the
+        n x n class matrix (0..5 as in the reference's buildCommonMtx)."""
+        n0 = max(self.pn, 1)
+        B = np.zeros(n0, dtype=np.uint32)
+        L = np.zeros(n0, dtype=np.uint32)
+        cls = np.zeros(1024 * 1024, dtype=np.uint8)
+        n, hop, l = c_uint32(), c_uint32(), c_uint32()
+        v = self.lib.kpo_root_block(ctypes.byref(self.p), self.k, self.q, pos, B.ctypes.data_as(POINTER(c_uint32)),
+                                    ctypes.byref(n), ctypes.byref(hop), L.ctypes.data_as(POINTER(c_uint32)),
+                                    ctypes.byref(l), cls.ctypes.data_as(POINTER(c_uint8)), cls.size)
+        if v < 0:
+            raise RuntimeError(f"kpo_root_block = {v}")
+        if v == 0:
+            return None
+        nn = n.value
+        return {"block": B[:nn].copy(), "hop": hop.value, "left": L[:l.value].copy(),
+                "cls": cls[:nn * nn].reshape(nn, nn).copy()}

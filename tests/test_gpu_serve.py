@@ -1,12 +1,15 @@
 """GPU tests of the served mode (kp_server_*): queries admitted into running
 epochs must give the same results as the plain one-graph runs, alone or in
 any mix, under slot and memory limits."""
+import json
+import os
 import sys
 import time
 
 import pytest
 
 import kplex
+from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
@@ -161,3 +164,38 @@ def test_server_rejects_lists_and_bad_params():
     with pytest.raises(kplex.KplexError):
         srv.submit(gid, 3, 5)
     srv.close()
+
+
+def test_server_served_workload_queries_equal_the_oracle():
+    # The served workload's fixture queries (the first 32 of its Poisson
+    # stream and the 8 largest graphs, n up to 198k, at k = 3, 4, 5) submitted
+    # a few at a time while epochs run: each result must equal the oracle's
+    # (tests/golden/oracle_served.json).
+    with open(os.path.join(GOLDEN, "oracle_served.json")) as f:
+        gold = json.load(f)["items"]
+    specs = kplex.served_graph_specs()
+    queries = kplex.served_fixture_queries()
+    assert len(gold) == len(queries) == 40
+    needed = sorted({gi for gi, _, _ in queries})
+    graphs = {gi: kplex.graph_from_spec(specs[gi]) for gi in needed}
+    srv = kplex.Server(max_inflight=16, slice_ms=1.0)
+    ids = {gi: srv.add_graph(graphs[gi]) for gi in needed}
+    say(f"served fixture queries: {len(queries)} over {len(needed)} graphs")
+    tickets, got = {}, {}
+    for i, (gi, k, q) in enumerate(queries):
+        assert (gold[i]["graph"], gold[i]["k"], gold[i]["q"]) == (gi, k, q)
+        tickets[srv.submit(ids[gi], k, q)] = i
+        if i % 4 == 3:
+            for t, r in srv.poll():
+                got[t] = r
+            time.sleep(0.002)
+    t0 = time.time()
+    while len(got) < len(queries):
+        for t, r in srv.poll():
+            got[t] = r
+        assert time.time() - t0 < 240, f"server did not finish: {len(got)}/{len(queries)}"
+        time.sleep(0.0005)
+    srv.close()
+    for t, i in tickets.items():
+        assert got[t]["status"] == kplex.KP_OK, (i, got[t]["status"])
+        same(got[t], gold[i])

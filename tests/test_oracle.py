@@ -150,3 +150,20 @@ def test_batch_fixtures_hold_for_the_quick_members():
         for x in ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l"):
             assert r[x] == gold[str(i)][x], (i, x)
     assert sum(v["count"] for v in gold.values()) == 1584447134
+
+
+def test_served_fixtures_hold_for_the_quick_queries():
+    # tests/golden/oracle_served.json against a fresh oracle run on the five
+    # quickest fixture queries
+    import kplex
+    with open(os.path.join(GOLDEN, "oracle_served.json")) as f:
+        gold = json.load(f)["items"]
+    specs = kplex.served_graph_specs()
+    queries = kplex.served_fixture_queries()
+    assert [(x["graph"], x["k"], x["q"]) for x in gold] == [tuple(q) for q in queries]
+    for i in sorted(range(len(gold)), key=lambda i: gold[i]["oracle_s"])[:5]:
+        gi, k, q = queries[i]
+        deg, nbr = kplex.graph_from_spec(specs[gi]).csr()
+        r = kpo.enumerate_plexes(kpo.OGraph.from_csr(deg, nbr), k, q, threads=4)
+        for x in ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l"):
+            assert r[x] == gold[i][x], (i, x)
