@@ -83,6 +83,8 @@ typedef struct kp_params {
 #define KP_FLAG_LIST      1u  /* return every maximal k-plex (original ids, sorted) */
 #define KP_FLAG_VALIDATE  2u  /* run kp_graph_validate first */
 #define KP_FLAG_VERBOSE   4u  /* print the reference's progress lines to stdout */
+#define KP_FLAG_GPU_PREP  8u  /* (q-k)-core peel, relabel and truss supports on the GPU; the truss
+                                 peel and the degeneracy order stay on the host (same result) */
 
 /* ---- result (inc/host_funcs.h:1144-1157) ---------------------------- */
 typedef struct kp_result {
@@ -147,6 +149,14 @@ int  kp_enumerate_noalloc(const kp_graph *g, const kp_params *p, kp_result *r);
  * CSR; orig and dseq must have room for g->n entries. */
 int  kp_preprocess(const kp_graph *g, int k, int q, int truss, kp_graph *out,
                    uint32_t *orig, int32_t *dseq);
+
+/* The same as kp_preprocess, with the (q-k)-core peel, the stable relabel
+ * and the truss supports (inc/host_funcs.h:16-122, :761-825) computed on the
+ * current HIP device; the one-sided truss peel (:463-559) and the degeneracy
+ * order (inc/degeneracy.h) stay on the host, whose orders they follow.  The
+ * output equals kp_preprocess's. */
+int  kp_preprocess_gpu(const kp_graph *g, int k, int q, int truss, kp_graph *out,
+                       uint32_t *orig, int32_t *dseq);
 
 /* ---- prepared (HBM-resident) graphs --------------------------------- */
 /* Split of kp_enumerate: kp_prepare runs the host preprocessing for (k, q,

@@ -66,6 +66,26 @@ extern "C" int kp_preprocess(const kp_graph *g, const int k, const int q, const 
     return KP_OK;
 }
 
+extern "C" int kp_preprocess_gpu(const kp_graph *g, const int k, const int q, const int truss, kp_graph *out,
+                                 uint32_t *orig, int32_t *dseq)
+{
+    if (!g || !out || !orig || !dseq) return KP_EINVAL;
+    kp_params p{};
+    p.k = k;
+    p.q = q;
+    p.truss = truss;
+    int rc = check_params(g, &p);
+    if (rc) return rc;
+    Prep pr;
+    rc = gpu_preprocess(g, k, q, truss, &pr);
+    if (rc) return rc;
+    *out = pr.g;
+    pr.g = kp_graph{};
+    std::copy(pr.orig.begin(), pr.orig.end(), orig);
+    std::copy(pr.dseq.begin(), pr.dseq.end(), dseq);
+    return KP_OK;
+}
+
 extern "C" int kp_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out)
 {
     if (!out) return KP_EINVAL;

@@ -531,6 +531,9 @@ __device__ __forceinline__ uint64_t rdl64(uint64_t x, uint32_t l)
 struct Cold {
     uint64_t t_start, t_mark, busy_t, idle_t;
     uint64_t alg, task0, bpt, frames, pops;
+    uint64_t plexes, ha, hb, tasks;   // accumulators of the current job (see Worker::tk)
+    uint64_t la;                      // the current root's left rows
+    uint32_t spills;
     uint64_t hungry_since;   // wall-clock tick at which the wave last became hungry
     uint32_t cjob, check_every;
 };
@@ -551,7 +554,6 @@ struct Worker {
     // current root (uniform)
     uint32_t cur = 0xffffffffu, n = 0, h = 0, l = 0;
     Cold *cold;
-    const uint64_t *LA = nullptr;
     // Rows of the vertices this lane owns (64j + lane) are kept in VGPRs.  The
     // search reads one class row per include step (the pivot's, a uniform
     // row), so beyond two words per row the class rows, the pivot's adjacency
@@ -565,9 +567,10 @@ struct Worker {
     uint32_t origR[RW];
     const uint64_t *Ab = nullptr, *C2b = nullptr, *C4b = nullptr;   // the root's arena rows (uniform)
     const uint32_t *OG = nullptr;
-    // accumulators (uniform); the bytes of the tasks are kept in Cold
-    uint64_t cnt = 0, ha = 0, hb = 0, tasks = 0;
-    uint32_t spills = 0;
+    // Branch tasks since the last fold into Cold::tasks (a 32-bit uniform
+    // counter: the job's accumulators live in LDS, which keeps the scalar
+    // registers of the frame loop free of state it touches once per plex)
+    uint32_t tk = 0;
     int err = 0;
 
     __device__ Worker(const SearchArgs &args, uint64_t *ring_, uint64_t *back_, Cold *cold_)
@@ -592,6 +595,9 @@ struct Worker {
         cold->t_mark = now;
         cold->busy_t = cold->idle_t = 0;
         cold->alg = cold->task0 = cold->bpt = cold->frames = cold->pops = 0;
+        cold->plexes = cold->ha = cold->hb = cold->tasks = 0;
+        cold->la = 0;
+        cold->spills = 0;
         cold->hungry_since = 0;
         cold->cjob = 0xffffffffu;
         // frames between checks of the error flag, the deadline and the pause
@@ -618,18 +624,13 @@ struct Worker {
         l = rfl(l);
         k = int(rfl(uint32_t(k)));
         q = int(rfl(uint32_t(q)));
-        LA = reinterpret_cast<const uint64_t *>(rfl64(reinterpret_cast<uint64_t>(LA)));
         if constexpr (!kRowRegs) {
             Ab = reinterpret_cast<const uint64_t *>(rfl64(reinterpret_cast<uint64_t>(Ab)));
             C2b = reinterpret_cast<const uint64_t *>(rfl64(reinterpret_cast<uint64_t>(C2b)));
             C4b = reinterpret_cast<const uint64_t *>(rfl64(reinterpret_cast<uint64_t>(C4b)));
             OG = reinterpret_cast<const uint32_t *>(rfl64(reinterpret_cast<uint64_t>(OG)));
         }
-        cnt = rfl64(cnt);
-        ha = rfl64(ha);
-        hb = rfl64(hb);
-        tasks = rfl64(tasks);
-        spills = rfl(spills);
+        tk = rfl(tk);
         err = int(rfl(uint32_t(err)));
     }
     static __device__ __forceinline__ void launder(Frame<W> &f)
@@ -783,7 +784,7 @@ struct Worker {
             bn++;
             rb = (rb + 1 == a.ring_cap) ? 0 : rb + 1;
             rn--;
-            spills++;
+            cold->spills = rfl(cold->spills) + 1;
         }
         store_slot(ring_slot(rn), meta, P, C, X, D, withD);
         rn++;
@@ -924,9 +925,17 @@ struct Worker {
         return true;
     }
 
+    // adds the tasks counted since the last fold to Cold::tasks
+    __device__ __forceinline__ void fold()
+    {
+        cold->tasks = rfl64(cold->tasks) + tk;
+        tk = 0;
+    }
     // task bytes of the current root so far go to Cold::alg
     __device__ void close_alg()
     {
+        fold();
+        const uint64_t tasks = rfl64(cold->tasks);
         cold->alg = rfl64(cold->alg) + (tasks - rfl64(cold->task0)) * rfl64(cold->bpt);
         cold->task0 = tasks;
     }
@@ -935,8 +944,11 @@ struct Worker {
     {
         // LDS values go through readfirstlane: the compiler then knows they
         // are wave-uniform and keeps what depends on them in scalar registers
+        fold();
         const uint32_t job = rfl(cold->cjob);
         const uint64_t alg = rfl64(cold->alg);
+        const uint64_t cnt = rfl64(cold->plexes), ha = rfl64(cold->ha), hb = rfl64(cold->hb);
+        const uint64_t tasks = rfl64(cold->tasks);
         if (job != 0xffffffffu && (cnt | tasks) && lane0()) {
             JobAcc *J = a.acc + job;
             atomicAdd(&J->count, (unsigned long long)cnt);
@@ -945,7 +957,7 @@ struct Worker {
             atomicAdd(&J->tasks, (unsigned long long)tasks);
             atomicAdd(&J->alg, (unsigned long long)alg);
         }
-        cnt = ha = hb = tasks = 0;
+        cold->plexes = cold->ha = cold->hb = cold->tasks = 0;
         cold->alg = 0;
         cold->task0 = 0;
     }
@@ -975,7 +987,8 @@ struct Worker {
         const uint64_t *A = reinterpret_cast<const uint64_t *>(rfl64(d->addr));
         const uint64_t *C2 = A + size_t(n) * W;
         const uint64_t *C4 = C2 + size_t(n) * W;
-        LA = C4 + size_t(n) * W;
+        const uint64_t *LA = C4 + size_t(n) * W;
+        cold->la = reinterpret_cast<uint64_t>(LA);
         const uint32_t *og = reinterpret_cast<const uint32_t *>(LA + size_t(l) * W);
         Ab = A;
         C2b = C2;
@@ -1133,6 +1146,7 @@ struct Worker {
             }
             if (__ballot(64u * t + ln < l && bad == 0 && c + k >= int(s) + 1)) return false;
         }
+        const uint64_t *LA = reinterpret_cast<const uint64_t *>(rfl64(cold->la));
         for (uint32_t t = 64u * LW; t < l; t += 64) {
             const uint32_t u = t + ln;
             bool ext = false;
@@ -1164,9 +1178,9 @@ struct Worker {
             }
         sa = rfl64(wsum64(sa));
         sb = rfl64(wsum64(sb));
-        cnt++;
-        ha += mix64(sa + uint64_t(s) * kp::HASH_LA);
-        hb += mix64(sb + uint64_t(s) * kp::HASH_LB);
+        cold->plexes = rfl64(cold->plexes) + 1;
+        cold->ha = rfl64(cold->ha) + mix64(sa + uint64_t(s) * kp::HASH_LA);
+        cold->hb = rfl64(cold->hb) + mix64(sb + uint64_t(s) * kp::HASH_LB);
         if constexpr (LISTS) {
             uint64_t base = 0;
             if (lane0()) base = atomicAdd((unsigned long long *)a.list_used, (unsigned long long)(s + 2));
@@ -1189,7 +1203,7 @@ struct Worker {
         }
     }
 
-    __device__ __forceinline__ void count_task() { tasks++; }
+    __device__ __forceinline__ void count_task() { tk++; }
 
     // ---- Branch frame (BNB :1516-1724) ----
     // The exclude child goes on the stack; the include child, when it exists,
@@ -1453,7 +1467,8 @@ struct Worker {
     // another path, which counts it from the start).
     __device__ void discard()
     {
-        cnt = ha = hb = tasks = 0;
+        tk = 0;
+        cold->plexes = cold->ha = cold->hb = cold->tasks = 0;
         cold->alg = 0;
         cold->task0 = 0;
         rn = bn = 0;
@@ -1718,6 +1733,7 @@ struct Worker {
             if (to_check == 0) {
                 const uint32_t ce = rfl(cold->check_every);
                 to_check = ce;
+                fold();
                 cold->frames = cold->frames + ce;
                 uint32_t stop = 0;
                 if (lane0()) {
@@ -1744,6 +1760,7 @@ struct Worker {
         close_alg();
         flush();
         const uint64_t busy_t = rfl64(cold->busy_t), idle_t = rfl64(cold->idle_t), pops = rfl64(cold->pops);
+        const uint32_t spills = rfl(cold->spills);
         const uint64_t frames = rfl64(cold->frames) + (rfl(cold->check_every) - to_check);
         uint32_t last = 0;
         if (lane0()) {
@@ -1959,6 +1976,12 @@ constexpr uint64_t kMonoBudget = 4096;
 constexpr uint64_t kDefaultQueueCap = 4194304;
 
 } // namespace
+
+int prep_graph(const kp_graph *g, const kp_params &p, Prep *out)
+{
+    if (p.flags & KP_FLAG_GPU_PREP) return gpu_preprocess(g, p.k, p.q, p.truss, out);
+    return preprocess(g, p.k, p.q, p.truss, out);
+}
 
 size_t packed_graph_bytes(const Prep &p)
 {
@@ -2993,7 +3016,7 @@ int dev_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out)
     int rc = h->E.init();
     if (rc) { delete h; return rc; }
     const double t0 = now_ms();
-    if ((rc = preprocess(g, p->k, p->q, p->truss, h->prep.get()))) { delete h; return rc; }
+    if ((rc = prep_graph(g, *p, h->prep.get()))) { delete h; return rc; }
     h->preprocess_ms = now_ms() - t0;
     const double t1 = now_ms();
     std::vector<GraphView> views;
@@ -3234,7 +3257,7 @@ int dev_enumerate(const kp_graph *g, const kp_params *p, kp_result *r)
     h->p = *p;
     h->prep.reset(new Prep());
     const double t0 = now_ms();
-    rc = preprocess(g, p->k, p->q, p->truss, h->prep.get());
+    rc = prep_graph(g, *p, h->prep.get());
     h->preprocess_ms = now_ms() - t0;
     if (!rc) {
         const double t1 = now_ms();
@@ -3413,10 +3436,14 @@ int dev_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result 
     std::vector<Prep> preps(n);
     std::vector<int> prc(n, KP_OK);
     std::vector<double> pms(n, 0);
-#pragma omp parallel for schedule(dynamic, 1)
+    // host preprocessing in parallel over the graphs; GPU preprocessing
+    // (KP_FLAG_GPU_PREP of p[0]) one graph after another
+    const bool on_gpu = (p[0].flags & KP_FLAG_GPU_PREP) != 0;
+#pragma omp parallel for schedule(dynamic, 1) if (!on_gpu)
     for (size_t i = 0; i < n; i++) {
         const double a = now_ms();
-        prc[i] = preprocess(g[i], p[i].k, p[i].q, p[i].truss, &preps[i]);
+        prc[i] = on_gpu ? gpu_preprocess(g[i], p[i].k, p[i].q, p[i].truss, &preps[i])
+                        : preprocess(g[i], p[i].k, p[i].q, p[i].truss, &preps[i]);
         pms[i] = now_ms() - a;
     }
     for (size_t i = 0; i < n; i++)

@@ -56,6 +56,9 @@ struct GraphView {
     uint32_t maxdeg = 0, d1max = 0;   // largest degree / largest later-neighbour count
 };
 
+// Host or GPU preprocessing (KP_FLAG_GPU_PREP in p.flags) of g for p's k, q, truss.
+int prep_graph(const kp_graph *g, const kp_params &p, Prep *out);
+
 // Packs preprocessed graphs into one device allocation (one host->HBM copy).
 size_t packed_graph_bytes(const Prep &p);
 int upload_graphs(const std::vector<const Prep *> &preps, DevBuf &buf, std::vector<GraphView> &views,

@@ -34,12 +34,25 @@ struct Prep {
     std::vector<uint32_t> orig;
     std::vector<int32_t> dseq, dpos;
     uint32_t pn_first = 0;   // vertices after the first core (before the truss)
+    double core_ms = 0, support_ms = 0, truss_ms = 0, degeneracy_ms = 0;   // host time per stage
     Prep() = default;
     Prep(const Prep &) = delete;
     Prep &operator=(const Prep &) = delete;
     ~Prep() { kp_graph_free(&g); }
 };
 int preprocess(const kp_graph *g, int k, int q, int truss, Prep *out);
+// the stages of preprocess, shared with the GPU preprocessing (kp_gprep.hip)
+int core_peel(const kp_graph *g, int bd, kp_graph *out, std::vector<uint32_t> &old_of_new);
+void truss_supports(const kp_graph *g, std::vector<int32_t> &tri);
+int truss_peel(kp_graph *g, int q, int k, std::vector<int32_t> &tri);
+void degeneracy_order(const kp_graph *g, std::vector<int32_t> &dseq, std::vector<int32_t> &dpos);
+// after the first core (out->g, old ids o1): the optional truss, with the
+// supports of out->g's entries in *tri (counted here when tri is null), the
+// second core, the degeneracy order
+int finish_preprocess(int k, int q, int truss, std::vector<uint32_t> &o1, std::vector<int32_t> *tri, Prep *out);
+// the same result as preprocess, with the first core, the relabel and the
+// truss supports computed on the current HIP device (kp_gprep.hip)
+int gpu_preprocess(const kp_graph *g, int k, int q, int truss, Prep *out);
 
 double now_ms();
 

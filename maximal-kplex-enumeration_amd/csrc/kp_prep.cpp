@@ -41,6 +41,8 @@ namespace {
 // adjacency entries below which preprocessing stays on the calling thread
 constexpr uint64_t kParallelEntries = 1u << 18;
 
+} // namespace
+
 // The bd-core is unique, so the parallel cascade below gives the same vertex
 // set as the reference's per-thread queues with CAS decrements.
 int core_peel(const kp_graph *g, int bd, kp_graph *out, std::vector<uint32_t> &old_of_new)
@@ -118,6 +120,8 @@ int core_peel(const kp_graph *g, int bd, kp_graph *out, std::vector<uint32_t> &o
     return KP_OK;
 }
 
+namespace {
+
 long find_entry(const kp_graph *g, uint32_t u, uint32_t v)
 {
     const uint32_t *b = g->nbr + g->off[u], *e = g->nbr + g->off[u + 1];
@@ -129,12 +133,14 @@ long find_entry(const kp_graph *g, uint32_t u, uint32_t v)
 // supports, seeds (i < v, support < q-2k) in (i, list) order, LIFO stack,
 // only the u->w and v->w entries decremented for a common neighbour w, and
 // an edge may be popped again after it is dead.
-int truss_prefilter(kp_graph *g, int q, int k)
+} // namespace
+
+// Support of every directed entry (u, v): |N(u) ∩ N(v)| by a sorted merge.
+void truss_supports(const kp_graph *g, std::vector<int32_t> &tri)
 {
     const uint32_t n = g->n;
     const uint64_t m = g->m;
-    const int thr = q - 2 * k;
-    std::vector<int32_t> tri(m);
+    tri.assign(m, 0);
     #pragma omp parallel for schedule(dynamic, 64) if (m > kParallelEntries)
     for (long i = 0; i < long(n); i++) {
         for (uint64_t p = g->off[i]; p < g->off[i + 1]; p++) {
@@ -150,6 +156,14 @@ int truss_prefilter(kp_graph *g, int q, int k)
             tri[p] = c;
         }
     }
+}
+
+// The truss peel with the supports already counted (tri is consumed).
+int truss_peel(kp_graph *g, int q, int k, std::vector<int32_t> &tri)
+{
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    const int thr = q - 2 * k;
     std::vector<std::pair<uint32_t, uint32_t>> stack;
     for (uint32_t i = 0; i < n; i++)
         for (uint64_t p = g->off[i]; p < g->off[i + 1]; p++)
@@ -238,18 +252,34 @@ void degeneracy_order(const kp_graph *g, std::vector<int32_t> &dseq, std::vector
     }
 }
 
-} // namespace
 
 int preprocess(const kp_graph *g, int k, int q, int truss, Prep *out)
 {
-    const int bd = q - k;
     std::vector<uint32_t> o1;
-    int rc = core_peel(g, bd, &out->g, o1);
+    const double t0 = now_ms();
+    int rc = core_peel(g, q - k, &out->g, o1);
     if (rc) return rc;
+    out->core_ms = now_ms() - t0;
+    return finish_preprocess(k, q, truss, o1, nullptr, out);
+}
+
+int finish_preprocess(int k, int q, int truss, std::vector<uint32_t> &o1, std::vector<int32_t> *tri, Prep *out)
+{
+    const int bd = q - k;
+    int rc;
     out->pn_first = out->g.n;
     if (truss) {
-        rc = truss_prefilter(&out->g, q, k);
+        double t = now_ms();
+        std::vector<int32_t> own;
+        if (!tri) {
+            truss_supports(&out->g, own);
+            tri = &own;
+        }
+        out->support_ms += now_ms() - t;
+        t = now_ms();
+        rc = truss_peel(&out->g, q, k, *tri);
         if (rc) return rc;
+        out->truss_ms = now_ms() - t;
         kp_graph g2{};
         std::vector<uint32_t> o2;
         rc = core_peel(&out->g, bd, &g2, o2);
@@ -260,7 +290,9 @@ int preprocess(const kp_graph *g, int k, int q, int truss, Prep *out)
         out->g = g2;
     }
     out->orig.swap(o1);
+    const double t = now_ms();
     degeneracy_order(&out->g, out->dseq, out->dpos);
+    out->degeneracy_ms = now_ms() - t;
     return KP_OK;
 }
 

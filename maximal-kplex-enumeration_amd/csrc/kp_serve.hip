@@ -152,7 +152,7 @@ int get_resident(kp_server *s, const Query &q, Resident **out, double *prep_ms)
     }
     auto r = std::make_unique<Resident>();
     const double t0 = now_ms();
-    int rc = preprocess(&s->graphs[q.graph], q.p.k, q.p.q, q.p.truss, &r->prep);
+    int rc = prep_graph(&s->graphs[q.graph], q.p, &r->prep);
     if (rc) return rc;
     r->prep_ms = now_ms() - t0;
     const double t1 = now_ms();
@@ -482,7 +482,7 @@ extern "C" int kp_server_preload(kp_server *s, const uint32_t *graph_ids, const 
             const size_t i = which[t];
             made[t] = std::make_unique<Resident>();
             const double t0 = now_ms();
-            rcs[t] = preprocess(&s->graphs[graph_ids[i]], p[i].k, p[i].q, p[i].truss, &made[t]->prep);
+            rcs[t] = prep_graph(&s->graphs[graph_ids[i]], p[i], &made[t]->prep);
             made[t]->prep_ms = now_ms() - t0;
         }
     };

@@ -29,6 +29,7 @@ KP_EHIP = -1000
 KP_FLAG_LIST = 1
 KP_FLAG_VALIDATE = 2
 KP_FLAG_VERBOSE = 4
+KP_FLAG_GPU_PREP = 8
 
 
 class KpGraph(ctypes.Structure):
@@ -99,6 +100,8 @@ SIGNATURES = {
     "kp_graph_relabel": (c_int, [POINTER(KpGraph), c_uint64, POINTER(KpGraph), POINTER(c_uint32)]),
     "kp_preprocess": (c_int, [POINTER(KpGraph), c_int, c_int, c_int, POINTER(KpGraph), POINTER(c_uint32),
                               POINTER(ctypes.c_int32)]),
+    "kp_preprocess_gpu": (c_int, [POINTER(KpGraph), c_int, c_int, c_int, POINTER(KpGraph), POINTER(c_uint32),
+                                  POINTER(ctypes.c_int32)]),
 }
 
 _lib = None
@@ -225,16 +228,18 @@ class Graph:
         return self
 
 
-def preprocess(graph, k, q, truss=0):
-    """Host preprocessing only: returns (peeled CSR deg, nbr, orig ids, degeneracy order)."""
+def preprocess(graph, k, q, truss=0, gpu=False):
+    """Preprocessing only (kp_preprocess on the host, or kp_preprocess_gpu with
+    gpu=True): returns (peeled CSR deg, nbr, orig ids, degeneracy order)."""
     lib = load()
     out = Graph()
     n = max(graph.n, 1)
     orig = np.zeros(n, dtype=np.uint32)
     dseq = np.zeros(n, dtype=np.int32)
-    _check(lib.kp_preprocess(ctypes.byref(graph.g), k, q, truss, ctypes.byref(out.g),
-                             orig.ctypes.data_as(POINTER(c_uint32)), dseq.ctypes.data_as(POINTER(ctypes.c_int32))),
-           "kp_preprocess")
+    fn = lib.kp_preprocess_gpu if gpu else lib.kp_preprocess
+    _check(fn(ctypes.byref(graph.g), k, q, truss, ctypes.byref(out.g),
+              orig.ctypes.data_as(POINTER(c_uint32)), dseq.ctypes.data_as(POINTER(ctypes.c_int32))),
+           "kp_preprocess_gpu" if gpu else "kp_preprocess")
     deg, nbr = out.csr()
     return deg, nbr, orig[:out.n], dseq[:out.n]
 
@@ -257,14 +262,15 @@ def make_params(k, q, thres=0.25, truss=0, flags=0, time_limit_s=0.0, queue_capa
 
 
 def enumerate_plexes(graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0, validate=False, roots=None,
-                     queue_capacity=0, pause_every=0):
+                     queue_capacity=0, pause_every=0, gpu_prep=False):
     """kp_enumerate: returns a dict with count, hash, tasks, timings (and plexes when listing).
     roots=(b, e) restricts the enumeration to roots at degeneracy positions [b, e).
     pause_every > 0 pauses each search launch after about that many frames and
     resumes it; thres * queue_capacity frames stay on the device, the rest spill
     to pinned host memory."""
     lib = load()
-    flags = (KP_FLAG_LIST if listing else 0) | (KP_FLAG_VALIDATE if validate else 0)
+    flags = (KP_FLAG_LIST if listing else 0) | (KP_FLAG_VALIDATE if validate else 0) | \
+        (KP_FLAG_GPU_PREP if gpu_prep else 0)
     p = make_params(k, q, thres, truss, flags, time_limit_s, queue_capacity, roots=roots, pause_every=pause_every)
     r = KpResult()
     _check(lib.kp_enumerate(ctypes.byref(graph.g), ctypes.byref(p), ctypes.byref(r)), "kp_enumerate")
