@@ -309,10 +309,10 @@ def full_graph_line(time_limit):
     tot = {"count": 0, "tasks": 0, "hash_a": 0, "hash_b": 0, "search_ms": 0.0, "valid_roots": 0}
     per = []
     t0 = time.perf_counter()
+    p = kplex.Prepared(g, k, q, thres, tr, time_limit_s=time_limit)
     for w in windows:
-        p = kplex.Prepared(g, k, q, thres, tr, time_limit_s=time_limit, roots=w)
+        p.set_roots(w)
         r = p.run()
-        p.close()
         for x in ("count", "tasks", "valid_roots"):
             tot[x] += r[x]
         tot["hash_a"] = (tot["hash_a"] + r["hash_a"]) % (1 << 64)
@@ -323,6 +323,7 @@ def full_graph_line(time_limit):
         log(f"[bench] full graph window {w}: count={r['count']} tasks={r['tasks']} gpu_ms={per[-1]['gpu_ms']:.0f} "
             f"(total {tot['count']} after {time.perf_counter() - t0:.1f} s)")
     wall = time.perf_counter() - t0
+    p.close()
     return {"workload": desc, "roots": end, "windows": len(windows), "count": tot["count"],
             "hash": f"{tot['hash_b']:016x}{tot['hash_a']:016x}", "tasks": tot["tasks"],
             "valid_roots": tot["valid_roots"], "gpu_s": tot["search_ms"] / 1e3, "wall_s": wall,

@@ -167,6 +167,10 @@ typedef struct kp_prepared kp_prepared;
 int  kp_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out);
 int  kp_run(kp_prepared *h, kp_result *r);
 uint64_t kp_prepared_bytes(const kp_prepared *h);
+/* Sets the root window of later kp_run calls (the meaning of kp_params.root_begin
+ * / root_end), so that a resident graph is enumerated window by window without
+ * preprocessing it again. */
+int  kp_set_window(kp_prepared *h, int64_t root_begin, int64_t root_end);
 /* Inspection hook for root-by-root parity: the decomposition and subgraph of
  * the root at degeneracy position pos of a prepared graph, as the GPU builds
  * them -- what the reference's decompose, calculateDegrees / fillNeighbors and

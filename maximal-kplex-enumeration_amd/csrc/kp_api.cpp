@@ -103,6 +103,15 @@ extern "C" int kp_run(kp_prepared *h, kp_result *r)
 
 extern "C" uint64_t kp_prepared_bytes(const kp_prepared *h) { return dev_bytes(h); }
 
+extern "C" int kp_set_window(kp_prepared *h, int64_t root_begin, int64_t root_end)
+{
+    if (!h || root_begin < 0 || root_end < 0 || (root_end && root_end < root_begin)) {
+        set_error("invalid root window [%lld, %lld)", (long long)root_begin, (long long)root_end);
+        return KP_EINVAL;
+    }
+    return dev_set_window(h, root_begin, root_end);
+}
+
 extern "C" int kp_autotune(kp_prepared *h, kp_tune *out)
 {
     if (!h || !out) return KP_EINVAL;

@@ -16,6 +16,7 @@ int  dev_root_subgraph(kp_prepared *h, int64_t pos, uint32_t *block, uint32_t *n
                        uint32_t *l, uint32_t left_cap, uint8_t *pairs, size_t pairs_cap);
 void dev_set_tune(const kp_tune *t);
 void dev_free(kp_prepared *h);
+int  dev_set_window(kp_prepared *h, int64_t root_begin, int64_t root_end);
 uint64_t dev_bytes(const kp_prepared *h);
 int  dev_info(char *buf, size_t len);
 const char *last_error();

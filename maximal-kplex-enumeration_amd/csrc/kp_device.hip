@@ -3005,6 +3005,13 @@ int dev_info(char *buf, size_t len)
 
 uint64_t dev_bytes(const kp_prepared *h) { return h ? h->gbuf.bytes + h->E.device_bytes() : 0; }
 
+int dev_set_window(kp_prepared *h, int64_t root_begin, int64_t root_end)
+{
+    h->p.root_begin = root_begin;
+    h->p.root_end = root_end;
+    return KP_OK;
+}
+
 void dev_free(kp_prepared *h) { delete h; }
 
 int dev_prepare(const kp_graph *g, const kp_params *p, kp_prepared **out)

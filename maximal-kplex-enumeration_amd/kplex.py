@@ -78,6 +78,7 @@ SIGNATURES = {
     "kp_prepare": (c_int, [POINTER(KpGraph), POINTER(KpParams), POINTER(c_void_p)]),
     "kp_run": (c_int, [c_void_p, POINTER(KpResult)]),
     "kp_prepared_bytes": (c_uint64, [c_void_p]),
+    "kp_set_window": (c_int, [c_void_p, ctypes.c_int64, ctypes.c_int64]),
     "kp_prepared_free": (None, [c_void_p]),
     "kp_root_subgraph": (c_int, [c_void_p, ctypes.c_int64, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
                                  POINTER(c_uint32), POINTER(c_uint32), c_uint32, POINTER(ctypes.c_uint8), c_size_t]),
@@ -328,6 +329,11 @@ class Prepared:
 
     def bytes(self):
         return self.lib.kp_prepared_bytes(self.h)
+
+    def set_roots(self, roots):
+        """kp_set_window: later runs enumerate root positions [b, e) (None = all)."""
+        b, e = roots if roots else (0, 0)
+        _check(self.lib.kp_set_window(self.h, b, e), "kp_set_window")
 
     def root_subgraph(self, pos):
         """kp_root_subgraph: the root at degeneracy position pos as the GPU

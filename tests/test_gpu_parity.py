@@ -138,6 +138,44 @@ def test_cl75k_window_against_golden(gold, window):
     same(r, gold[key], ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l"))
 
 
+def test_cl75k_windows_past_9000_against_oracle_chunks():
+    # the Chung-Lu 75k workload past position 9000, window by window, against
+    # the oracle's chunk fixtures (tests/golden/oracle_cl75k_chunks.json); then
+    # one window spanning all the recorded chunks against their sum (windows
+    # partition the result: counts, task counts and hashes add up)
+    path = os.path.join(GOLDEN, "oracle_cl75k_chunks.json")
+    if not os.path.exists(path):
+        pytest.skip("no recorded oracle chunks")
+    with open(path) as f:
+        chunks = json.load(f)["items"]
+    wins = sorted((tuple(int(x) for x in key.split("_")), v) for key, v in chunks.items())
+    g, k, q, _, tr = kplex.workload_cl75k()
+    p = kplex.Prepared(g, k, q, truss=tr, time_limit_s=LIMIT)
+    try:
+        for (b, e), want in wins:
+            say(f"cl75k window [{b},{e})")
+            p.set_roots((b, e))
+            same(p.run(), want, ("count", "hash", "tasks", "valid_roots", "pn", "pm", "max_n", "max_l"))
+        lo = wins[0][0][0]
+        hi = lo
+        for (b, e), _ in wins:
+            if b != hi:
+                break
+            hi = e
+        span = [v for (b, e), v in wins if e <= hi]
+        say(f"cl75k window [{lo},{hi}) against the sum of {len(span)} chunks")
+        p.set_roots((lo, hi))
+        r = p.run()
+        m = (1 << 64) - 1
+        assert r["count"] == sum(v["count"] for v in span)
+        assert r["tasks"] == sum(v["tasks"] for v in span)
+        ha = sum(int(v["hash"][16:], 16) for v in span) & m
+        hb = sum(int(v["hash"][:16], 16) for v in span) & m
+        assert (r["hash_a"], r["hash_b"]) == (ha, hb)
+    finally:
+        p.close()
+
+
 def test_root_windows_partition_the_result():
     # windows [0,a) and [a,end) add up to the whole enumeration, in count, hash and tasks
     g = kplex.Graph.er_planted(1200, 0.01, 6, 30, 3, 9)
