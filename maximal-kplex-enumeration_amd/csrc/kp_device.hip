@@ -2830,10 +2830,20 @@ int Engine::launch_epoch(const SearchOpts &o)
         order.emplace_back(-double(C.htodo.size() - C.todo_done + C.nseed + C.spill_n), c);
     }
     std::stable_sort(order.begin(), order.end());
+    // The primary / secondary split needs roots to tell the classes apart.
+    // An epoch that resumes paused frames runs every class at one block per
+    // CU: a pause dumps every wave's stack, and at most the carry budget of
+    // them come back per epoch, so a wide grid would dump more frames at the
+    // next pause than the budget brings back (measured on the Chung-Lu 320k
+    // window with 1,024 frames per class kept on the device: 4.9 s at one
+    // block per CU, 12.8 s on full grids).
+    bool resumed = false;
+    for (const auto &oc : order) resumed |= cls[oc.second].nseed + cls[oc.second].spill_n > 0;
+    const bool demote = !resumed && !order.empty() && order[0].second > 0;
     for (size_t oi = 0; oi < order.size() && !rc; oi++) {
         const int c = order[oi].second;
         if (hipStreamWaitEvent(cls[c].stream, ev, 0) != hipSuccess) rc = KP_EHIP;
-        else rc = launch_class_w(*this, c, o, oi > 0 && order[0].second > 0);
+        else rc = launch_class_w(*this, c, o, resumed || (oi > 0 && demote));
         if (!rc && std::getenv("KP_SERIAL_CLASSES") && hipStreamSynchronize(cls[c].stream) != hipSuccess) rc = KP_EHIP;
     }
     (void)hipEventDestroy(ev);
@@ -3122,11 +3132,25 @@ int run_pool(Engine &E, const std::vector<JobIn> &jobs, const kp_params &P0, std
     SearchOpts o = search_opts(E, P0, E.device, orc);
     if (orc) return orc;
     HIPCHK(hipEventRecord(e0, E.stream));
+    // the per-root pass takes the jobs on small graphs; the roots it leaves,
+    // and every root of the other jobs, go to the pool
     std::vector<std::vector<uint32_t>> left;
     std::vector<JobIn> jobs2 = jobs;
-    if (mono_budget && !o.want_list) {
-        if ((rc = E.run_roots(jobs, mono_budget, o.limit_ticks, left))) return rc;
-        for (size_t i = 0; i < jobs2.size(); i++) jobs2[i].pos_list = &left[i];
+    if (mono_budget && !o.want_list && P0.pause_every == 0) {
+        std::vector<JobIn> small;
+        std::vector<size_t> which;
+        for (size_t i = 0; i < jobs.size(); i++)
+            if (jobs[i].g.pn <= kMonoMaxPn) {
+                small.push_back(jobs[i]);
+                which.push_back(i);
+            }
+        std::vector<std::vector<uint32_t>> lsmall;
+        if (!small.empty() && (rc = E.run_roots(small, mono_budget, o.limit_ticks, lsmall))) return rc;
+        left.assign(jobs.size(), {});
+        for (size_t t = 0; t < which.size(); t++) {
+            left[which[t]].swap(lsmall[t]);
+            jobs2[which[t]].pos_list = &left[which[t]];
+        }
     }
     HIPCHK(hipEventRecord(em, E.stream));
     std::vector<JobStats> st;
@@ -3204,8 +3228,7 @@ int dev_run(kp_prepared *h, kp_result *res)
     float dms = 0, sms = 0;
     // Small graphs take the per-root pass first: one launch walks the roots
     // whose subtrees stay under the frame budget; the rest go to the pool.
-    const uint64_t mono = (h->gv.pn <= kMonoMaxPn && h->p.pause_every == 0) ? kMonoBudget : 0;
-    if ((rc = run_pool(h->E, {j}, h->p, rv, dms, sms, mono))) return rc;
+    if ((rc = run_pool(h->E, {j}, h->p, rv, dms, sms, kMonoBudget))) return rc;
     res->preprocess_ms = h->preprocess_ms;
     res->upload_ms = h->upload_ms;
     res->pn_first = h->prep->pn_first;
@@ -3473,7 +3496,7 @@ int dev_batch(const kp_graph *const *g, const kp_params *p, size_t n, kp_result 
     std::vector<kp_result *> rv(n);
     for (size_t i = 0; i < n; i++) rv[i] = &r[i];
     float dms = 0, sms = 0;
-    if ((rc = run_pool(E, jobs, p[0], rv, dms, sms))) return rc;
+    if ((rc = run_pool(E, jobs, p[0], rv, dms, sms, kMonoBudget))) return rc;
     for (size_t i = 0; i < n; i++) {
         r[i].preprocess_ms = pms[i];
         r[i].pn_first = preps[i].pn_first;
