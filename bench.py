@@ -114,7 +114,7 @@ def cpu_baseline(graph, k, q, truss, target_s, window=None):
                        f"preprocessing included per sample chunk")}
 
 
-def batched_vs_plain(ngraphs, time_limit):
+def batched_vs_plain(ngraphs, time_limit, chunks=1):
     """The many-graph workload (Chung-Lu exponent 2.3, average degree 10, n
     log-uniform in [200, 20000], 1-20 planted 25-50-vertex communities at 0.9,
     k in {2,3,4}, q = 3k+8), first `ngraphs` graphs of the 4,096: every graph
@@ -136,14 +136,21 @@ def batched_vs_plain(ngraphs, time_limit):
     plain_s = time.perf_counter() - t0
     log(f"[bench] batched: plain {plain_s:.2f} s")
     t0 = time.perf_counter()
-    batch = kplex.enumerate_batch(graphs, [dict(k=k, q=q, time_limit_s=time_limit) for k, q in params])
+    batch = []
+    step = (len(graphs) + chunks - 1) // chunks
+    for b in range(0, len(graphs), step):
+        batch += kplex.enumerate_batch(graphs[b:b + step],
+                                       [dict(k=k, q=q, time_limit_s=time_limit) for k, q in params[b:b + step]])
+        if chunks > 1:
+            log(f"[bench] batched: graphs [{b},{min(len(graphs), b + step)}) in one batch, "
+                f"{time.perf_counter() - t0:.1f} s so far")
     batch_s = time.perf_counter() - t0
-    log(f"[bench] batched: one batch {batch_s:.2f} s")
+    log(f"[bench] batched: {chunks} batch call(s) {batch_s:.2f} s")
     for a, b in zip(plain, batch):
         for x in ("count", "hash", "tasks", "valid_roots"):
             assert a[x] == b[x], ("batched result differs", x, a[x], b[x])
     count = sum(r["count"] for r in batch)
-    return {"graphs": len(graphs), "plexes": count, "tasks": sum(r["tasks"] for r in batch),
+    return {"graphs": len(graphs), "batch_calls": chunks, "plexes": count, "tasks": sum(r["tasks"] for r in batch),
             "plain_s": plain_s, "batched_s": batch_s, "plain_plexes_per_s": count / plain_s,
             "batched_plexes_per_s": count / batch_s, "batched_vs_plain": plain_s / batch_s,
             "plain_gpu_s": sum(r["decompose_ms"] + r["search_ms"] for r in plain) / 1e3,
@@ -168,20 +175,12 @@ def served_vs_plain(duration_s, rate, time_limit, seed=7):
     (continuous batching); latency = completion - arrival.  Then the same
     queries run one after another through the plain entry point
     (kp_enumerate per query).  Results must be identical query by query."""
-    import numpy as np
     import kplex
-    specs = kplex.batch_graph_specs(count=256, nmax=200000, seed=4, kset=(3, 4, 5), qfun=lambda k: 5 * k + 10)
+    specs = kplex.served_graph_specs()
     t0 = time.perf_counter()
     graphs = gen_graphs(specs)
     gen_s = time.perf_counter() - t0
-    rng = np.random.default_rng(seed)
-    arrivals, queries = [], []
-    t = rng.exponential(1.0 / rate)
-    while t < duration_s:
-        k = int(rng.choice([3, 4, 5]))
-        arrivals.append(t)
-        queries.append((int(rng.integers(len(graphs))), k, 5 * k + 10))
-        t += rng.exponential(1.0 / rate)
+    arrivals, queries = kplex.served_queries(rate, duration_s, seed, len(graphs))
     log(f"[bench] served: {len(graphs)} graphs generated in {gen_s:.1f} s; {len(queries)} queries at {rate}/s "
         f"over {duration_s} s")
     srv = kplex.Server(max_inflight=128, slice_ms=2.0)
@@ -221,6 +220,8 @@ def served_vs_plain(duration_s, rate, time_limit, seed=7):
         r = kplex.enumerate_plexes(graphs[gi], k, q, time_limit_s=time_limit)
         for x in ("count", "hash", "tasks", "valid_roots"):
             assert r[x] == done[i][1][x], ("served result differs", i, x, r[x], done[i][1][x])
+        if (i + 1) % 500 == 0:
+            log(f"[bench] served: plain {i + 1}/{len(queries)} queries in {time.perf_counter() - t0:.1f} s")
     plain_s = time.perf_counter() - t0
     log(f"[bench] served: plain one-by-one {plain_s:.2f} s")
     return {"graphs": len(graphs), "queries": len(queries), "offered_per_s": rate, "arrival_window_s": duration_s,
@@ -362,6 +363,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=48,
                     help="graphs of the many-graph workload for the batched-vs-plain line (0 = skip)")
+    ap.add_argument("--batch-chunks", type=int, default=1,
+                    help="split the batch into this many consecutive kp_enumerate_batch calls (long runs print per call)")
     ap.add_argument("--served-seconds", type=float, default=1.5,
                     help="arrival window of the served stream at --served-rate queries/s (0 = skip)")
     ap.add_argument("--served-rate", type=float, default=200.0)
@@ -463,7 +466,7 @@ def main():
     if not args.no_spill_line:
         out["spill"] = spill_line(args.time_limit)
     if args.batch_graphs > 0:
-        out["batched"] = batched_vs_plain(args.batch_graphs, args.time_limit)
+        out["batched"] = batched_vs_plain(args.batch_graphs, args.time_limit, args.batch_chunks)
     if args.served_seconds > 0:
         out["served"] = served_vs_plain(args.served_seconds, args.served_rate, args.time_limit)
     if args.full_graph:

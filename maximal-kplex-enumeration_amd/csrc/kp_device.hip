@@ -2025,12 +2025,15 @@ int upload_graphs(const std::vector<const Prep *> &preps, DevBuf &buf, std::vect
         v.orig = reinterpret_cast<const uint32_t *>(put(P.orig.data(), 4 * n));
         v.pn = uint32_t(n);
         v.pm = m;
-        uint32_t maxdeg = 0, d1 = 0;
-        for (uint32_t u = 0; u < G.n; u++) {
-            maxdeg = std::max(maxdeg, G.deg[u]);
-            uint32_t later = 0;
-            for (uint64_t e = G.off[u]; e < G.off[u + 1]; e++) later += P.dpos[G.nbr[e]] > P.dpos[u];
-            d1 = std::max(d1, later);
+        uint32_t maxdeg = P.maxdeg, d1 = P.d1max;
+        if (!P.sizes_known) {
+            maxdeg = d1 = 0;
+            for (uint32_t u = 0; u < G.n; u++) {
+                maxdeg = std::max(maxdeg, G.deg[u]);
+                uint32_t later = 0;
+                for (uint64_t e = G.off[u]; e < G.off[u + 1]; e++) later += P.dpos[G.nbr[e]] > P.dpos[u];
+                d1 = std::max(d1, later);
+            }
         }
         v.maxdeg = maxdeg;
         v.d1max = d1;

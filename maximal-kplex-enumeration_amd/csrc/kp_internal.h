@@ -35,6 +35,8 @@ struct Prep {
     std::vector<int32_t> dseq, dpos;
     uint32_t pn_first = 0;   // vertices after the first core (before the truss)
     double core_ms = 0, support_ms = 0, truss_ms = 0, degeneracy_ms = 0;   // host time per stage
+    uint32_t maxdeg = 0, d1max = 0;   // largest degree / largest later-neighbour count (when sizes_known)
+    bool sizes_known = false;
     Prep() = default;
     Prep(const Prep &) = delete;
     Prep &operator=(const Prep &) = delete;
@@ -45,7 +47,9 @@ int preprocess(const kp_graph *g, int k, int q, int truss, Prep *out);
 int core_peel(const kp_graph *g, int bd, kp_graph *out, std::vector<uint32_t> &old_of_new);
 void truss_supports(const kp_graph *g, std::vector<int32_t> &tri);
 int truss_peel(kp_graph *g, int q, int k, std::vector<int32_t> &tri);
-void degeneracy_order(const kp_graph *g, std::vector<int32_t> &dseq, std::vector<int32_t> &dpos);
+// max_later (optional) receives the largest number of later neighbours of a vertex
+void degeneracy_order(const kp_graph *g, std::vector<int32_t> &dseq, std::vector<int32_t> &dpos,
+                      uint32_t *max_later = nullptr);
 // after the first core (out->g, old ids o1): the optional truss, with the
 // supports of out->g's entries in *tri (counted here when tri is null), the
 // second core, the degeneracy order

@@ -101,12 +101,15 @@ int dev_enumerate_noalloc(const kp_graph *g, const kp_params *p, kp_result *r)
     int64_t pb = 0, pe = 0;
     root_window(pn, *p, pb, pe);
     const size_t S = size_t(pe - pb);
-    uint32_t maxdeg = 0, d1 = 0;
-    for (uint32_t u = 0; u < pn; u++) {
-        maxdeg = std::max(maxdeg, G.deg[u]);
-        uint32_t later = 0;
-        for (uint64_t e = G.off[u]; e < G.off[u + 1]; e++) later += prep.dpos[G.nbr[e]] > prep.dpos[u];
-        d1 = std::max(d1, later);
+    uint32_t maxdeg = prep.maxdeg, d1 = prep.d1max;
+    if (!prep.sizes_known) {
+        maxdeg = d1 = 0;
+        for (uint32_t u = 0; u < pn; u++) {
+            maxdeg = std::max(maxdeg, G.deg[u]);
+            uint32_t later = 0;
+            for (uint64_t e = G.off[u]; e < G.off[u + 1]; e++) later += prep.dpos[G.nbr[e]] > prep.dpos[u];
+            d1 = std::max(d1, later);
+        }
     }
     const uint32_t pn1 = std::max<uint32_t>(pn, 1);
     const uint32_t lcap = std::min<uint32_t>(pn1, std::max<uint32_t>(2048, 2 * maxdeg + 1024));

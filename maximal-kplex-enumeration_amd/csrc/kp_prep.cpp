@@ -43,10 +43,64 @@ constexpr uint64_t kParallelEntries = 1u << 18;
 
 } // namespace
 
+namespace {
+
+// The same core and relabel on the calling thread, for small graphs: one
+// stack of peeled vertices, one pass each for the new degrees and lists.
+int core_peel_small(const kp_graph *g, int bd, kp_graph *out, std::vector<uint32_t> &old_of_new)
+{
+    const uint32_t n = g->n;
+    std::vector<int32_t> res(n);
+    std::vector<uint32_t> map(n), stack;
+    stack.reserve(n);
+    for (uint32_t i = 0; i < n; i++) {
+        res[i] = int32_t(g->deg[i]);
+        map[i] = int64_t(g->deg[i]) >= bd ? 0u : UINT32_MAX;
+        if (map[i]) stack.push_back(i);
+    }
+    while (!stack.empty()) {
+        const uint32_t u = stack.back();
+        stack.pop_back();
+        for (uint64_t p = g->off[u]; p < g->off[u + 1]; p++) {
+            const uint32_t v = g->nbr[p];
+            if (map[v] == 0 && res[v]-- == bd) {
+                map[v] = UINT32_MAX;
+                stack.push_back(v);
+            }
+        }
+    }
+    old_of_new.clear();
+    uint64_t pm = 0;
+    for (uint32_t i = 0; i < n; i++)
+        if (map[i] == 0) {
+            map[i] = uint32_t(old_of_new.size());
+            old_of_new.push_back(i);
+            pm += uint64_t(res[i]);   // survivors keep exactly their surviving neighbours
+        }
+    const uint32_t pn = uint32_t(old_of_new.size());
+    int rc = graph_alloc(out, pn, pm);
+    if (rc) return rc;
+    uint64_t w = 0;
+    out->off[0] = 0;
+    for (uint32_t i = 0; i < pn; i++) {
+        const uint32_t o = old_of_new[i];
+        for (uint64_t p = g->off[o]; p < g->off[o + 1]; p++) {
+            const uint32_t v = map[g->nbr[p]];
+            if (v != UINT32_MAX) out->nbr[w++] = v;
+        }
+        out->deg[i] = uint32_t(w - out->off[i]);
+        out->off[i + 1] = w;
+    }
+    return KP_OK;
+}
+
+} // namespace
+
 // The bd-core is unique, so the parallel cascade below gives the same vertex
 // set as the reference's per-thread queues with CAS decrements.
 int core_peel(const kp_graph *g, int bd, kp_graph *out, std::vector<uint32_t> &old_of_new)
 {
+    if (g->m <= kParallelEntries) return core_peel_small(g, bd, out, old_of_new);
     const uint32_t n = g->n;
     std::vector<int32_t> res(n);
     std::vector<uint8_t> alive(n);
@@ -208,7 +262,7 @@ int truss_peel(kp_graph *g, int q, int k, std::vector<int32_t> &tri)
 // Min-degree peeling order with the reference's bucket heap: inserts go to
 // the head of a bucket (LIFO), pop takes the head of the lowest non-empty
 // bucket, a decrement moves the vertex to the head of its new bucket.
-void degeneracy_order(const kp_graph *g, std::vector<int32_t> &dseq, std::vector<int32_t> &dpos)
+void degeneracy_order(const kp_graph *g, std::vector<int32_t> &dseq, std::vector<int32_t> &dpos, uint32_t *max_later)
 {
     const int n = int(g->n);
     dseq.assign(n, 0);
@@ -237,6 +291,8 @@ void degeneracy_order(const kp_graph *g, std::vector<int32_t> &dseq, std::vector
     for (int i = 0; i < n; i++) {
         while (head[min_key] == n) ++min_key;
         const int u = head[min_key];
+        // the key at the pop is u's number of later neighbours
+        if (max_later && uint32_t(min_key) > *max_later) *max_later = uint32_t(min_key);
         head[min_key] = nxt[u];
         if (head[min_key] != n) prv[head[min_key]] = n;
         dpos[u] = i;
@@ -291,7 +347,11 @@ int finish_preprocess(int k, int q, int truss, std::vector<uint32_t> &o1, std::v
     }
     out->orig.swap(o1);
     const double t = now_ms();
-    degeneracy_order(&out->g, out->dseq, out->dpos);
+    out->d1max = 0;
+    degeneracy_order(&out->g, out->dseq, out->dpos, &out->d1max);
+    out->maxdeg = 0;
+    for (uint32_t u = 0; u < out->g.n; u++) out->maxdeg = std::max(out->maxdeg, out->g.deg[u]);
+    out->sizes_known = true;
     out->degeneracy_ms = now_ms() - t;
     return KP_OK;
 }
