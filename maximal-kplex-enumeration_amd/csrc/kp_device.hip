@@ -1580,7 +1580,7 @@ struct Worker {
                         si = rfl(si);
                         if (si < a.nseed) {
                             active_self = true;
-                            load_slot(a.seed + size_t(si) * FW, f);
+                            load_slot(si < a.nseed_dev ? a.seed + size_t(si) * FW : a.hseed + size_t(si - a.nseed_dev) * FW, f);
                             have = true;
                             got = true;
                         }
@@ -2478,8 +2478,15 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     // primary, 198 ms at four blocks and 166 ms at two; a 4,357-vertex graph
     // of the many-graph workload, 36 ms at four one-word blocks and 195 ms at
     // seven).  Batches and served epochs with many roots keep the full grid.
+    // A class with nothing but paused frames left resumes on the grid it
+    // paused on: a pause changes nothing about the work that remains
+    // (measured on the Chung-Lu 320k window paused every 200M frames: with
+    // the resumed classes re-ranked, the three-word class ran at one block
+    // per CU and set the pace of every epoch).
     const uint64_t pending = uint64_t(C.htodo.size() - C.todo_done) + C.nseed + C.spill_n;
+    const bool resume_only = C.nseed + C.spill_n > 0 && C.todo_done == C.htodo.size() && C.blocks_per_cu > 0;
     if (T.blocks_per_cu > 0) nb = std::min(nb, T.blocks_per_cu);
+    else if (resume_only) nb = std::min(nb, C.blocks_per_cu);
     else if (secondary) nb = 1;
     else if (pending < uint64_t(nb) * uint64_t(E.cus) * 4u) nb = std::min(nb, W >= 2 ? 2 : 4);
     const uint32_t nwg = uint32_t(nb) * uint32_t(E.cus);
@@ -2520,8 +2527,13 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     sa.desc = C.desc.as<RootDesc>();
     sa.todo = C.todo.as<uint32_t>();
     sa.ntodo = uint32_t(ntodo);
+    // resumed frames: the device seeds, then every spilled frame read in place
+    // from pinned host memory
+    const size_t nhost = std::min<size_t>(C.spill_n, 0x7fffffffu - C.nseed);
     sa.seed = C.nseed ? C.seed[C.cur_seed].as<uint64_t>() : nullptr;
-    sa.nseed = C.nseed;
+    sa.hseed = nhost ? C.spill_dev : nullptr;
+    sa.nseed_dev = C.nseed;
+    sa.nseed = C.nseed + uint32_t(nhost);
     sa.dump = o.pausable ? C.dump.as<uint64_t>() : nullptr;
     sa.dump_cap = dump_cap;
     sa.acc = E.acc.as<JobAcc>();
@@ -2551,6 +2563,7 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     C.back = back;
     C.dump_cap = dump_cap;
     C.launch_todo = uint32_t(ntodo);
+    C.launch_seeds = sa.nseed;
     C.blocks_per_cu = nb;
     return KP_OK;
 }
@@ -2581,27 +2594,41 @@ int grow_spill(ClassState &C, size_t need, size_t fbytes)
 {
     if (need <= C.spill_cap) return KP_OK;
     const size_t cap = std::max<size_t>(need, 2 * C.spill_cap);
-    void *p = nullptr;
-    if (hipHostMalloc(&p, cap * fbytes, hipHostMallocDefault) != hipSuccess) {
+    void *p = nullptr, *d = nullptr;
+    // mapped and coherent: search kernels read spilled frames in place
+    if (hipHostMalloc(&p, cap * fbytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
         set_error("pinned host allocation of %zu bytes failed", cap * fbytes);
         return KP_ENOMEM;
+    }
+    if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
+        (void)hipHostFree(p);
+        set_error("pinned host frames have no device address");
+        return KP_EHIP;
     }
     if (C.spill_n) std::memcpy(p, C.spill, C.spill_n * fbytes);
     if (C.spill) (void)hipHostFree(C.spill);
     C.spill = static_cast<uint64_t *>(p);
+    C.spill_dev = static_cast<const uint64_t *>(d);
     C.spill_cap = cap;
     return KP_OK;
 }
 
-// Pending frames after a launch: `D` dumped frames plus the resumed frames
-// [ns, nseed) it did not take.  Up to `budget` of them become the next
-// launch's resumed frames on the device; the rest are spilled to pinned host
-// memory (a stack), from which later launches refill.  All copies run on the
-// side stream.
+// Pending frames after a launch: `D` dumped frames plus the resumed frames it
+// did not take.  The launch offered the device seeds [0, nseed) and then the
+// host frames [0, launch_seeds - nseed); `ns` of them, in that order, were
+// taken.  Up to `budget` pending frames become the next launch's device seeds;
+// the rest stay in (or go to) pinned host memory, which the next launch reads
+// in place.  All copies run on the side stream.
 int carry_frames(Engine &E, ClassState &C, size_t fbytes, uint32_t D, uint32_t ns, uint64_t budget, EpochStats &es)
 {
     const size_t R = size_t(std::max<uint64_t>(1, std::min<uint64_t>(budget, 0x7fffffffu)));
-    const uint32_t U = C.nseed > ns ? C.nseed - ns : 0;
+    const uint32_t cd = std::min(ns, C.nseed);                       // device seeds taken
+    const size_t ch = std::min<size_t>(ns - cd, C.spill_n);          // host frames taken
+    if (ch) {
+        std::memmove(C.spill, reinterpret_cast<const char *>(C.spill) + ch * fbytes, (C.spill_n - ch) * fbytes);
+        C.spill_n -= ch;
+    }
+    const uint32_t U = C.nseed - cd;
     const size_t want = std::min<size_t>(R, size_t(D) + U + C.spill_n);
     DevBuf &nb = C.seed[1 - C.cur_seed];
     int rc;
@@ -2614,7 +2641,7 @@ int carry_frames(Engine &E, ClassState &C, size_t fbytes, uint32_t D, uint32_t n
     if (kd) HIPCHK(hipMemcpyAsync(dst, dump, kd * fbytes, hipMemcpyDeviceToDevice, E.side));
     fill = kd;
     const size_t ku = std::min<size_t>(U, R - fill);
-    if (ku) HIPCHK(hipMemcpyAsync(dst + fill * fbytes, old + size_t(ns) * fbytes, ku * fbytes, hipMemcpyDeviceToDevice, E.side));
+    if (ku) HIPCHK(hipMemcpyAsync(dst + fill * fbytes, old + size_t(cd) * fbytes, ku * fbytes, hipMemcpyDeviceToDevice, E.side));
     fill += ku;
     const size_t sd = D - kd, su = U - ku;
     if (sd + su) {
@@ -2622,7 +2649,7 @@ int carry_frames(Engine &E, ClassState &C, size_t fbytes, uint32_t D, uint32_t n
         char *hs = reinterpret_cast<char *>(C.spill) + C.spill_n * fbytes;
         if (sd) HIPCHK(hipMemcpyAsync(hs, dump + kd * fbytes, sd * fbytes, hipMemcpyDeviceToHost, E.side));
         if (su)
-            HIPCHK(hipMemcpyAsync(hs + sd * fbytes, old + (size_t(ns) + ku) * fbytes, su * fbytes, hipMemcpyDeviceToHost, E.side));
+            HIPCHK(hipMemcpyAsync(hs + sd * fbytes, old + (size_t(cd) + ku) * fbytes, su * fbytes, hipMemcpyDeviceToHost, E.side));
         C.spill_n += sd + su;
         es.spilled += sd + su;
         E.total_spilled += sd + su;
@@ -2833,20 +2860,13 @@ int Engine::launch_epoch(const SearchOpts &o)
         order.emplace_back(-double(C.htodo.size() - C.todo_done + C.nseed + C.spill_n), c);
     }
     std::stable_sort(order.begin(), order.end());
-    // The primary / secondary split needs roots to tell the classes apart.
-    // An epoch that resumes paused frames runs every class at one block per
-    // CU: a pause dumps every wave's stack, and at most the carry budget of
-    // them come back per epoch, so a wide grid would dump more frames at the
-    // next pause than the budget brings back (measured on the Chung-Lu 320k
-    // window with 1,024 frames per class kept on the device: 4.9 s at one
-    // block per CU, 12.8 s on full grids).
-    bool resumed = false;
-    for (const auto &oc : order) resumed |= cls[oc.second].nseed + cls[oc.second].spill_n > 0;
-    const bool demote = !resumed && !order.empty() && order[0].second > 0;
+    // A resumed epoch offers every pending frame (device seeds and spilled
+    // frames read in place), so it is launched like a fresh one.
+    const bool demote = !order.empty() && order[0].second > 0;
     for (size_t oi = 0; oi < order.size() && !rc; oi++) {
         const int c = order[oi].second;
         if (hipStreamWaitEvent(cls[c].stream, ev, 0) != hipSuccess) rc = KP_EHIP;
-        else rc = launch_class_w(*this, c, o, resumed || (oi > 0 && demote));
+        else rc = launch_class_w(*this, c, o, oi > 0 && demote);
         if (!rc && std::getenv("KP_SERIAL_CLASSES") && hipStreamSynchronize(cls[c].stream) != hipSuccess) rc = KP_EHIP;
     }
     (void)hipEventDestroy(ev);
@@ -2965,9 +2985,9 @@ int Engine::finish_epoch(const SearchOpts &o, EpochStats &es)
         es.frames += hs.frames;
         es.donated += hs.donated;
         const uint32_t started = std::min(hs.next_root, C.launch_todo);
-        const uint32_t ns = std::min(hs.next_seed, C.nseed);
+        const uint32_t ns = std::min(hs.next_seed, C.launch_seeds);
         C.todo_done += started;
-        if (!hs.pause && (started < C.launch_todo || ns < C.nseed || hs.dump_used)) {
+        if (!hs.pause && (started < C.launch_todo || ns < C.launch_seeds || hs.dump_used)) {
             if (!first_err) {
                 set_error("search kernel (class W=%d) ended with work left", kClassW[c]);
                 first_err = KP_EINVAL;

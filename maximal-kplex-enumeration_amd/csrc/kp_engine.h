@@ -100,11 +100,13 @@ struct ClassState {
     uint32_t maxn = 0;
     int cur_seed = 0;
     uint32_t nseed = 0;                 // resumed frames in seed[cur_seed]
-    uint64_t *spill = nullptr;          // pinned host frames (a stack)
+    uint64_t *spill = nullptr;          // pinned host frames, mapped and coherent
+    const uint64_t *spill_dev = nullptr;  // the same frames' device address
     size_t spill_cap = 0, spill_n = 0;  // in frames
     // last launch
     bool launched = false;
     uint32_t nwaves = 0, ring = 0, back = 0, dump_cap = 0, launch_todo = 0;
+    uint32_t launch_seeds = 0;          // device seeds, then host frames [0, launch_seeds - nseed)
     hipEvent_t done_ev = nullptr;
     bool has_work() const { return todo_done < htodo.size() || nseed > 0 || spill_n > 0; }
     ~ClassState();

@@ -105,8 +105,9 @@ struct SearchArgs {
     const RootDesc *desc;      // class root table
     const uint32_t *todo;      // roots to start, in order (indices into desc)
     uint32_t ntodo;
-    const uint64_t *seed;      // frames to resume
-    uint32_t nseed;
+    const uint64_t *seed;      // frames to resume: seed[0, nseed_dev) on the device,
+    const uint64_t *hseed;     // then hseed[0, nseed - nseed_dev) in mapped pinned host memory
+    uint32_t nseed, nseed_dev;
     uint64_t *dump;            // frames written on pause
     uint32_t dump_cap;
     JobAcc *acc;               // per-job accumulators
