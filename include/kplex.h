@@ -85,6 +85,8 @@ typedef struct kp_params {
 #define KP_FLAG_VERBOSE   4u  /* print the reference's progress lines to stdout */
 #define KP_FLAG_GPU_PREP  8u  /* (q-k)-core peel, relabel and truss supports on the GPU; the truss
                                  peel and the degeneracy order stay on the host (same result) */
+#define KP_FLAG_NO_NARROW 16u /* keep every frame in its root's size class: no re-rooting of
+                                 frames of wide blocks into the one-word class (same result) */
 
 /* ---- result (inc/host_funcs.h:1144-1157) ---------------------------- */
 typedef struct kp_result {
@@ -113,6 +115,8 @@ typedef struct kp_result {
                               called at :749 and :823) */
     int32_t  status;       /* kp_server_poll: KP_OK, or the KP_E* code that failed this query
                               (its counts are then zero) */
+    uint64_t narrowed;     /* frames of blocks wider than 64 vertices whose live set fit one
+                              word and that were re-rooted into the one-word class */
 } kp_result;
 /*
  * Hash: for each maximal k-plex S of original vertex ids

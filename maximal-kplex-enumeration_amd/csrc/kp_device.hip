@@ -31,12 +31,14 @@
 #include "kp_engine.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 namespace kpd {
@@ -552,8 +554,9 @@ struct Worker {
     // two-level stack (uniform): the newest frames in the LDS ring, older ones in HBM
     uint32_t rb = 0, rn = 0, bb = 0, bn = 0;
     // current root (uniform)
-    uint32_t cur = 0xffffffffu, n = 0, h = 0, l = 0;
+    uint32_t cur = 0xffffffffu, n = 0, l = 0;
     Cold *cold;
+    uint32_t *sel;   // this wave's 64-entry LDS scratch (narrowing)
     // Rows of the vertices this lane owns (64j + lane) are kept in VGPRs.  The
     // search reads one class row per include step (the pivot's, a uniform
     // row), so beyond two words per row the class rows, the pivot's adjacency
@@ -572,9 +575,10 @@ struct Worker {
     // registers of the frame loop free of state it touches once per plex)
     uint32_t tk = 0;
     int err = 0;
+    uint32_t nar_full = 0;   // set once a derived-root reservation failed
 
-    __device__ Worker(const SearchArgs &args, uint64_t *ring_, uint64_t *back_, Cold *cold_)
-        : a(args), S(args.sched), ring(ring_), back(back_), cold(cold_)
+    __device__ Worker(const SearchArgs &args, uint64_t *ring_, uint64_t *back_, Cold *cold_, uint32_t *sel_ = nullptr)
+        : a(args), S(args.sched), ring(ring_), back(back_), cold(cold_), sel(sel_)
     {
         ln = __lane_id();
         k = 0;
@@ -620,7 +624,6 @@ struct Worker {
         bn = rfl(bn);
         cur = rfl(cur);
         n = rfl(n);
-        h = rfl(h);
         l = rfl(l);
         k = int(rfl(uint32_t(k)));
         q = int(rfl(uint32_t(q)));
@@ -632,6 +635,7 @@ struct Worker {
         }
         tk = rfl(tk);
         err = int(rfl(uint32_t(err)));
+        nar_full = rfl(nar_full);
     }
     static __device__ __forceinline__ void launder(Frame<W> &f)
     {
@@ -965,12 +969,23 @@ struct Worker {
     // ---- root cache: this lane's rows of the current root ----
     // MayFlush: the root may belong to another job than the accumulators
     // (then they are flushed first); callers that pass false have checked
+    // root table entry of a frame's root: the class table, or (one-word class)
+    // a derived-root buffer
+    __device__ __forceinline__ const RootDesc *desc_of(uint32_t ridx) const
+    {
+        if (W == 1 && (ridx & DERIVED_BIT))
+            return a.ddesc[(ridx >> DERIVED_SHIFT) & (NDERIVED_BUF - 1)] + (ridx & DERIVED_INDEX_MASK);
+        return a.desc + ridx;
+    }
     template <bool MayFlush>
     __device__ void load_root(uint32_t ridx)
     {
-        const RootDesc *d = a.desc + ridx;
+        const RootDesc *d = desc_of(ridx);
         n = rfl(d->n);
-        h = rfl(d->h);
+        // a derived root (narrowed from a wider block) keeps its block's size
+        // in h for the per-task bytes; h is not needed after push_root
+        const uint32_t dh = rfl(d->h);
+        const uint32_t blk_n = (dh & 0x80000000u) ? (dh & 0x7fffffffu) : n;
         l = rfl(d->l);
         const uint32_t job = rfl(d->job);
         const uint32_t kq = rfl(*reinterpret_cast<const uint32_t *>(&d->k));
@@ -983,7 +998,7 @@ struct Worker {
                 cold->cjob = job;
             }
         }
-        cold->bpt = 2ull * (5ull * n + 16ull);
+        cold->bpt = 2ull * (5ull * blk_n + 16ull);
         const uint64_t *A = reinterpret_cast<const uint64_t *>(rfl64(d->addr));
         const uint64_t *C2 = A + size_t(n) * W;
         const uint64_t *C4 = C2 + size_t(n) * W;
@@ -1398,6 +1413,186 @@ struct Worker {
         return true;
     }
 
+    // ---- narrowing (wide classes): re-root a frame whose live set fits one word ----
+    // Bits of a W-word row r at the positions of the uniform set V, packed in
+    // order (V holds at most 64 vertices).
+    static __device__ __forceinline__ uint64_t gather_bits(const uint64_t *r, const uint64_t *V)
+    {
+        uint64_t out = 0;
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            uint64_t mw = V[w];
+            const uint64_t x = r[w];   // a per-lane row in memory, read word by word
+            while (mw) {
+                const uint32_t b = uint32_t(__builtin_ctzll(mw));
+                mw &= mw - 1;
+                out |= ((x >> b) & 1ull) << t;
+                t++;
+            }
+        }
+        return out;
+    }
+    // lane predicate: vertex s (per lane) is in the uniform set Sx
+    static __device__ __forceinline__ bool has_v(const uint64_t *Sx, uint32_t s)
+    {
+        uint64_t word = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) word = (s >> 6) == uint32_t(w) ? Sx[w] : word;
+        return (word >> (s & 63)) & 1ull;
+    }
+    // Left vertex u can still extend a plex S of this subtree (P ⊆ S ⊆ U =
+    // P ∪ C ∪ D, |S| >= q) only if it misses at most k-1 vertices of P, is
+    // adjacent to every saturated vertex of P (they stay saturated in S), and
+    // has at least q+1-k neighbours in U: maximal() tests exactly these
+    // conditions on S, so a left vertex that fails one never extends.
+    __device__ __forceinline__ bool left_keeps(const uint64_t *LA, uint32_t u, const uint64_t *P, const uint64_t *U,
+                                               const uint64_t *SatP, uint32_t ps) const
+    {
+        const size_t uu = size_t(min(u, l - 1));
+        int c = 0, cp = 0;
+        uint64_t bad = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            const uint64_t x = LA[uu * W + w];
+            c += __popcll(x & U[w]);
+            cp += __popcll(x & P[w]);
+            bad |= SatP[w] & ~x;
+        }
+        return u < l && bad == 0 && cp + k >= int(ps) + 1 && c + k >= q + 1;
+    }
+    // Called before a frame is processed.  The excluded set X is tested only
+    // for extension (an x in X blocks a record when S + x is a k-plex, and is
+    // kept at an include step only when compatible with the new P), so an x
+    // that is incompatible with P (then with every P' ⊇ P) or has fewer than
+    // q+1-k neighbours in U (then never extends a plex of size >= q) changes no
+    // test; such vertices are dropped for good.  If P ∪ C ∪ D ∪ X then fits
+    // one word, the frame becomes a derived root of the one-word class with
+    // those vertices in block order (pivot and k-search choices depend only on
+    // that order, so the subtree is the same) and returns true.
+    __device__ bool try_narrow(Frame<W> &f)
+    {
+        const NarrowArgs &N = a.nar;
+        uint64_t U[W];
+        uint32_t cu = 0, cd = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            U[w] = f.P[w] | f.C[w] | f.D[w];
+            cu += __popcll(U[w]);
+            cd += __popcll(f.C[w] | f.D[w]);
+        }
+        const uint32_t ps = popc(f.P);
+        if (cu > 64 || ps + cd < uint32_t(q) + N.min_slack) return false;
+        uint64_t OK[W], V[W], SatP[W];
+        compat_mask(f.P, ps, OK);
+        uint32_t m = 0;
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            f.X[j] &= OK[j] & __ballot(int(deg_in(j, U)) + k >= q + 1);
+            V[j] = U[j] | f.X[j];
+            m += __popcll(V[j]);
+            SatP[j] = f.P[j] & __ballot(int(deg_in(j, f.P)) + k == int(ps));
+        }
+        if (m > 64) return false;
+        const uint64_t *LA = reinterpret_cast<const uint64_t *>(rfl64(cold->la));
+        uint32_t nl = 0;
+        for (uint32_t t = 0; t < l; t += 64) nl += uint32_t(__popcll(__ballot(left_keeps(LA, t + ln, f.P, U, SatP, ps))));
+        // subgraph words: adj, cls2, cls4 rows (one word each), left rows, original ids
+        const uint64_t words = 3ull * m + nl + (m + 1) / 2;
+        uint64_t base = 0, idx = 0;
+        uint32_t ok = 0;
+        if (lane0()) {
+            base = atomicAdd(&N.ctr[0], (unsigned long long)words);
+            if (base + words <= N.arena_cap) {
+                idx = atomicAdd(&N.ctr[1], 1ull);
+                ok = idx < N.cap;
+            }
+        }
+        if (!rfl(ok)) {
+            // no more narrowing in this launch (the next reservations would
+            // fail too); the class pauses if asked to, so that the next epoch
+            // resumes it with an empty buffer
+            nar_full = 1;
+            if (N.pause_when_full && lane0()) __hip_atomic_store(&S->pause, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        base = rfl64(base);
+        idx = rfl64(idx);
+        uint64_t *A = N.arena + base;
+        // lane t takes the vertex of compact index t
+        uint32_t off = 0;
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            if (V[j]) {
+                if (inmask(V[j])) sel[off + __popcll(V[j] & lt_mask())] = 64u * j + ln;
+                off += __popcll(V[j]);
+            }
+        }
+        const uint32_t s = sel[min(ln, m - 1)];
+        // the three rows of vertex s, gathered word by word
+        uint64_t ga = 0, g2 = 0, g4 = 0;
+        {
+            uint32_t t = 0;
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                const uint64_t xa = Ab[size_t(s) * W + w], x2 = C2b[size_t(s) * W + w], x4 = C4b[size_t(s) * W + w];
+                uint64_t mw = V[w];
+                while (mw) {
+                    const uint32_t b = uint32_t(__builtin_ctzll(mw));
+                    mw &= mw - 1;
+                    ga |= ((xa >> b) & 1ull) << t;
+                    g2 |= ((x2 >> b) & 1ull) << t;
+                    g4 |= ((x4 >> b) & 1ull) << t;
+                    t++;
+                }
+            }
+        }
+        const uint32_t og = OG[s];
+        uint32_t *ogd = reinterpret_cast<uint32_t *>(A + 3ull * m + nl);
+        if (ln < m) {
+            A[ln] = ga;
+            A[m + ln] = g2;
+            A[2 * m + ln] = g4;
+            ogd[ln] = og;
+        }
+        uint32_t pos = 0;
+        for (uint32_t t = 0; t < l; t += 64) {
+            const uint32_t u = t + ln;
+            const bool kp = left_keeps(LA, u, f.P, U, SatP, ps);
+            const uint64_t bm = __ballot(kp);
+            if (bm) {
+                const uint64_t gl = gather_bits(LA + size_t(min(u, l - 1)) * W, V);
+                if (kp) A[3ull * m + pos + __popcll(bm & lt_mask())] = gl;
+                pos += uint32_t(__popcll(bm));
+            }
+        }
+        const uint64_t live = m == 64 ? ~0ull : ((1ull << m) - 1ull);
+        const bool in = ln < m;
+        const uint64_t P1 = __ballot(in && has_v(f.P, s)), C1 = __ballot(in && has_v(f.C, s));
+        const uint64_t X1 = __ballot(in && has_v(f.X, s)), D1 = __ballot(in && has_v(f.D, s));
+        if (lane0()) {
+            RootDesc d{};
+            d.addr = reinterpret_cast<uint64_t>(A);
+            d.n = m;
+            d.h = 0x80000000u | n;
+            d.l = nl;
+            d.slot = 0;
+            d.job = cold->cjob;
+            d.k = uint16_t(k);
+            d.q = uint16_t(q);
+            N.desc[idx] = d;
+            uint64_t *F = N.frames + idx * uint64_t(FrameLayout<1>::FW);
+            F[0] = make_meta(N.tag | uint32_t(idx), f.type, f.r);
+            F[1] = P1 & live;
+            F[2] = C1 & live;
+            F[3] = X1 & live;
+            F[4] = D1 & live;
+            F[5] = 0;
+        }
+        asm volatile("");
+        return true;
+    }
+
     __device__ void push_root(uint32_t ridx)
     {
         const RootDesc *d = a.desc + ridx;
@@ -1430,6 +1625,9 @@ struct Worker {
             pz = __hip_atomic_load(&S->pause, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (!pz && a.host_pause &&
                 __hip_atomic_load(a.host_pause, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0)
+                pz = 1;
+            if (!pz && a.class_pause &&
+                __hip_atomic_load(a.class_pause, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0)
                 pz = 1;
             if (!pz && a.pause_frames && new_frames) {
                 const uint64_t old = atomicAdd((unsigned long long *)&S->epoch_frames, (unsigned long long)new_frames);
@@ -1523,6 +1721,7 @@ struct Worker {
     // while hungry waves outnumber the queued frames.
     __device__ void run()
     {
+        if (lane0()) atomicAdd(&S->started, 1u);
         bool active_self = false, hungry_self = false, have = false, paused = false;
         uint32_t to_donate = a.donate_period, backoff = 1;   // frames left until the next donation check
         uint32_t to_check = rfl(cold->check_every);   // frames left until the next check
@@ -1641,10 +1840,12 @@ struct Worker {
                         // class whose remaining work is thin sheds waves; the
                         // margin keeps enough hungry waves for donations to
                         // double the busy set each round.
-                        if (backoff >= 16 && ns >= a.nseed && nr >= a.ntodo &&
+                        // A class whose own grid is not yet all resident, or
+                        // that runs alone (retire_ticks 0), keeps its waves.
+                        if (a.retire_ticks && backoff >= 16 && ns >= a.nseed && nr >= a.ntodo &&
                             wall_clock64() - rfl64(cold->hungry_since) > a.retire_ticks) {
                             uint32_t retire = 0;
-                            if (lane0()) {
+                            if (lane0() && __hip_atomic_load(&S->started, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.nwaves) {
                                 const uint32_t ac = __hip_atomic_load(&S->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                                 const uint32_t qc2 = __hip_atomic_load(&S->q_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                                 const uint32_t old = atomicSub(&S->hungry, 1u);
@@ -1688,11 +1889,14 @@ struct Worker {
                 if (f.root != cur) {
                     // a root of another job: its accumulators are flushed
                     // outside the burst (the frame stays pending in f)
-                    if (rfl(a.desc[f.root].job) != rfl(cold->cjob)) break;
+                    if (rfl(desc_of(f.root)->job) != rfl(cold->cjob)) break;
                     load_root<false>(f.root);
                 }
                 Frame<W> nf;
-                have = (f.type == FT_KSEARCH) ? ksearch(f, nf) : branch(f, nf);
+                bool narrowed = false;
+                if constexpr (W >= 2) narrowed = a.nar.cap != 0 && !nar_full && try_narrow(f);
+                if (narrowed) have = false;
+                else have = (f.type == FT_KSEARCH) ? ksearch(f, nf) : branch(f, nf);
                 nfr++;
                 if (err) break;
                 if (have) f = nf;
@@ -1800,6 +2004,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1 ? 4 
 {
     extern __shared__ uint64_t lds[];
     __shared__ Cold s_cold[4];
+    __shared__ uint32_t s_sel[4][W >= 2 ? 64 : 1];
     constexpr int FW = FrameLayout<W>::FW;
     // the wave index is uniform within a wave; readfirstlane tells the
     // compiler, so that everything derived from it (ring, stack, LDS state,
@@ -1809,7 +2014,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1 ? 4 
     if (gw >= a.nwaves) return;
     uint64_t *ring = lds + size_t(wv) * a.ring_cap * FW;
     uint64_t *back = a.gstack ? a.gstack + size_t(gw) * a.back_cap * FW : nullptr;
-    Worker<W, LW, LISTS> wk(a, ring, back, &s_cold[wv]);
+    Worker<W, LW, LISTS> wk(a, ring, back, &s_cold[wv], s_sel[wv]);
     wk.run();
 }
 
@@ -2122,9 +2327,10 @@ Engine::~Engine()
 uint64_t Engine::device_bytes() const
 {
     const DevBuf *all[] = {&jobs, &acc, &blk, &lft, &info, &slot_job, &slot_pos, &ws_cnt, &ws_b1c, &ws_a1,
-                           &bdesc, &wr_extra, &list, &list_used};
+                           &bdesc, &wr_extra, &list, &list_used, &nar_ctr};
     uint64_t s = 0;
     for (const DevBuf *b : all) s += b->bytes;
+    for (const NarBuf &b : nar) s += b.arena.bytes + b.desc.bytes + b.frames.bytes;
     for (const DevBuf *b : segs) s += b->bytes;
     for (const auto &c : cls) {
         const DevBuf *cb[] = {&c.q, &c.qseq, &c.sched, &c.stack, &c.subq, &c.desc, &c.todo, &c.seed[0], &c.seed[1], &c.dump};
@@ -2189,6 +2395,7 @@ void Engine::reset_tables()
         c.launched = false;
     }
     segs_used = 0;
+    for (auto &b : nar) b.in_use = false;
 }
 
 uint64_t *Engine::arena_alloc(uint64_t words)
@@ -2445,7 +2652,7 @@ int Engine::stage(const std::vector<JobIn> &jv, std::vector<JobStats> &st,
 namespace {
 
 template <int W, int LW>
-int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
+int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary, bool alone)
 {
     ClassState &C = E.cls[c];
     constexpr int FW = FrameLayout<W>::FW;
@@ -2461,6 +2668,7 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     int nb = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, dyn));
     if (nb < 1) nb = 1;
+    C.nb_max = nb;
     // Classes launch together on their own streams, and idle waves of a class
     // that has nothing left to start retire, so that blocks of the other
     // classes take their slots.  Untuned, when the epoch's primary class (the
@@ -2485,10 +2693,19 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     // per CU and set the pace of every epoch).
     const uint64_t pending = uint64_t(C.htodo.size() - C.todo_done) + C.nseed + C.spill_n;
     const bool resume_only = C.nseed + C.spill_n > 0 && C.todo_done == C.htodo.size() && C.blocks_per_cu > 0;
-    if (T.blocks_per_cu > 0) nb = std::min(nb, T.blocks_per_cu);
-    else if (resume_only) nb = std::min(nb, C.blocks_per_cu);
-    else if (secondary) nb = 1;
-    else if (pending < uint64_t(nb) * uint64_t(E.cus) * 4u) nb = std::min(nb, W >= 2 ? 2 : 4);
+    // (a resumed class keeps its demotion: it runs on the grid it paused on)
+    if (T.blocks_per_cu > 0) {
+        nb = std::min(nb, T.blocks_per_cu);
+        C.demoted = false;
+    } else if (resume_only) {
+        nb = std::min(nb, C.blocks_per_cu);
+    } else if (secondary) {
+        nb = 1;
+        C.demoted = nb < C.nb_max;
+    } else {
+        C.demoted = false;
+        if (pending < uint64_t(nb) * uint64_t(E.cus) * 4u) nb = std::min(nb, W >= 2 ? 2 : 4);
+    }
     const uint32_t nwg = uint32_t(nb) * uint32_t(E.cus);
     SearchArgs sa{};
     sa.nwaves = nwg * 4;
@@ -2498,7 +2715,9 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     sa.donate_period = T.donate_period ? T.donate_period : 256u;
     sa.donate_target = std::max<uint32_t>(16, sa.nwaves / 4);
     sa.retire_margin = std::max<uint32_t>(32, sa.nwaves / 16);
-    sa.retire_ticks = E.ticks_per_ms * 10;
+    // a class alone on the chip keeps its waves: retiring them frees slots
+    // for no one, and waves that stay are the ones donations can reach
+    sa.retire_ticks = alone ? 0 : E.ticks_per_ms * 10;
     const size_t nslots = size_t(sa.qcap) * NSUBQ;
     int rc;
     if ((rc = C.q.ensure(nslots * FW * 8)) || (rc = C.qseq.ensure(nslots * 8)) || (rc = C.sched.ensure(sizeof(Sched))) ||
@@ -2547,8 +2766,24 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     sa.list_used = E.list_used.as<uint64_t>();
     sa.limit_ticks = o.limit_ticks;
     sa.host_pause = o.host_pause ? E.dpause : nullptr;
+    sa.class_pause = (o.regrid && E.dpause) ? E.dpause + 1 + c : nullptr;
+    if (sa.class_pause) __atomic_store_n(&E.hpause[1 + c], 0u, __ATOMIC_RELEASE);
     sa.pause_frames = o.pause_frames;
     sa.t0 = E.t0.as<unsigned long long>();
+    if (W >= 2 && E.nar_cur >= 0 && W >= o.narrow_from) {
+        Engine::NarBuf &B = E.nar[E.nar_cur];
+        sa.nar.arena = B.arena.as<uint64_t>();
+        sa.nar.arena_cap = B.arena.bytes / 8;
+        sa.nar.ctr = E.nar_ctr.as<unsigned long long>();
+        sa.nar.desc = B.desc.as<RootDesc>();
+        sa.nar.frames = B.frames.as<uint64_t>();
+        sa.nar.cap = E.nar_cap;
+        sa.nar.tag = DERIVED_BIT | (uint32_t(E.nar_cur) << DERIVED_SHIFT);
+        sa.nar.min_slack = o.narrow_min_slack;
+        sa.nar.pause_when_full = o.narrow_pause && o.pausable;
+    }
+    if (W == 1)
+        for (uint32_t b = 0; b < NDERIVED_BUF; b++) sa.ddesc[b] = E.nar[b].desc.as<RootDesc>();
     HIPCHK(hipMemsetAsync(sa.sched, 0, sizeof(Sched), C.stream));
     HIPCHK(hipMemsetAsync(sa.subq, 0, sizeof(SubQ) * NSUBQ, C.stream));
     hipLaunchKernelGGL(iota_kernel, dim3(uint32_t((nslots + 255) / 256)), dim3(256), 0, C.stream, sa.qseq,
@@ -2568,16 +2803,16 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary)
     return KP_OK;
 }
 
-int launch_class_w(Engine &E, int c, const SearchOpts &o, bool secondary = false)
+int launch_class_w(Engine &E, int c, const SearchOpts &o, bool secondary = false, bool alone = false)
 {
     switch (kClassW[c]) {
-    case 1: return launch_class<1, 2>(E, c, o, secondary);
-    case 2: return launch_class<2, 2>(E, c, o, secondary);
-    case 3: return launch_class<3, 2>(E, c, o, secondary);
-    case 4: return launch_class<4, 1>(E, c, o, secondary);
-    case 6: return launch_class<6, 0>(E, c, o, secondary);
-    case 8: return launch_class<8, 0>(E, c, o, secondary);
-    case 16: return launch_class<16, 0>(E, c, o, secondary);
+    case 1: return launch_class<1, 2>(E, c, o, secondary, alone);
+    case 2: return launch_class<2, 2>(E, c, o, secondary, alone);
+    case 3: return launch_class<3, 2>(E, c, o, secondary, alone);
+    case 4: return launch_class<4, 1>(E, c, o, secondary, alone);
+    case 6: return launch_class<6, 0>(E, c, o, secondary, alone);
+    case 8: return launch_class<8, 0>(E, c, o, secondary, alone);
+    case 16: return launch_class<16, 0>(E, c, o, secondary, alone);
     }
     return KP_EINVAL;
 }
@@ -2841,6 +3076,36 @@ int Engine::run_roots(const std::vector<JobIn> &jv, uint64_t budget, uint64_t li
 
 int Engine::launch_epoch(const SearchOpts &o)
 {
+    if (o.regrid) {
+        int rc = enable_host_pause();
+        if (rc) return rc;
+    }
+    // Narrowing: the epoch fills a free derived-root buffer (none free: no
+    // narrowing in this epoch); its counters restart here.
+    nar_cur = -1;
+    if (o.narrow_from > 0) {
+        bool wide = false;
+        for (int c = 1; c < kNumClass; c++) wide |= kClassW[c] >= o.narrow_from && cls[c].has_work();
+        int pick = -1;
+        for (int b = 0; b < int(NDERIVED_BUF) && pick < 0 && wide; b++)
+            if (!nar[b].in_use) pick = b;
+        if (pick >= 0) {
+            NarBuf &B = nar[pick];
+            if (!nar_cap) {
+                nar_cap = kNarrowRoots;
+                if (const char *e = std::getenv("KP_NARROW_ROOTS"))
+                    nar_cap = std::min<uint32_t>(DERIVED_INDEX_MASK, uint32_t(std::max(1, std::atoi(e))));
+            }
+            size_t arena = kNarrowArenaBytes;
+            if (const char *e = std::getenv("KP_NARROW_ARENA_MB")) arena = size_t(std::max(1, std::atoi(e))) << 20;
+            int rc;
+            if ((rc = nar_ctr.ensure(256)) || (rc = B.desc.ensure(size_t(nar_cap) * sizeof(RootDesc))) ||
+                (rc = B.frames.ensure(size_t(nar_cap) * FrameWords(1) * 8)) || (rc = B.arena.ensure(arena)))
+                return rc;
+            if ((rc = fill_words(stream, nar_ctr.p, 2, 0))) return rc;
+            nar_cur = pick;
+        }
+    }
     // the class streams start after everything queued on the main stream
     HIPCHK(zero_t0(*this, stream));
     hipEvent_t ev;
@@ -2866,7 +3131,7 @@ int Engine::launch_epoch(const SearchOpts &o)
     for (size_t oi = 0; oi < order.size() && !rc; oi++) {
         const int c = order[oi].second;
         if (hipStreamWaitEvent(cls[c].stream, ev, 0) != hipSuccess) rc = KP_EHIP;
-        else rc = launch_class_w(*this, c, o, oi > 0 && demote);
+        else rc = launch_class_w(*this, c, o, oi > 0 && demote, order.size() == 1);
         if (!rc && std::getenv("KP_SERIAL_CLASSES") && hipStreamSynchronize(cls[c].stream) != hipSuccess) rc = KP_EHIP;
     }
     (void)hipEventDestroy(ev);
@@ -2887,6 +3152,40 @@ int Engine::enable_host_pause()
     HIPCHK(hipHostGetDevicePointer(&d, p, 0));
     dpause = static_cast<uint32_t *>(d);
     return KP_OK;
+}
+
+int Engine::watch_epoch(const SearchOpts &o)
+{
+    if (!o.regrid || !hpause) return KP_OK;
+    for (;;) {
+        bool running = false, finished = false, blocker = false;
+        int cand[kNumClass], nc = 0;
+        for (int c = 0; c < kNumClass; c++) {
+            ClassState &C = cls[c];
+            if (!C.launched) continue;
+            const hipError_t q = hipEventQuery(C.done_ev);
+            if (q == hipSuccess) {
+                finished = true;
+                continue;
+            }
+            if (q != hipErrorNotReady) return KP_OK;   // finish_epoch reports the failure
+            running = true;
+            if (C.regrid_sent) continue;
+            if (C.demoted) cand[nc++] = c;
+            else blocker = true;
+        }
+        if (!running) return KP_OK;
+        // Every class still running was demoted to one block per CU beside a
+        // primary class that has drained: pause them, and the next epoch
+        // launches the one with the most pending work on the whole chip.
+        if (finished && !blocker && nc) {
+            for (int i = 0; i < nc; i++) {
+                __atomic_store_n(&hpause[1 + cand[i]], 1u, __ATOMIC_RELEASE);
+                cls[cand[i]].regrid_sent = true;
+            }
+        }
+        std::this_thread::sleep_for(std::chrono::microseconds(100));
+    }
 }
 
 bool Engine::epoch_done()
@@ -2918,6 +3217,7 @@ int Engine::count_pending(std::vector<uint64_t> &per_job)
 int Engine::time_class(int c, const SearchOpts &o, float *ms)
 {
     ClassState &C = cls[c];
+    nar_cur = -1;   // a timed class runs alone, without narrowing
     const size_t saved_done = C.todo_done;
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
@@ -2966,6 +3266,10 @@ int Engine::finish_epoch(const SearchOpts &o, EpochStats &es)
         const Sched &hs = cg.second;
         ClassState &C = cls[c];
         C.launched = false;
+        if (C.regrid_sent) {
+            C.regrid_sent = false;
+            C.blocks_per_cu = 0;   // relaunched by the ordinary rules, not on the grid it paused on
+        }
         if (std::getenv("KP_STATS"))
             std::fprintf(stderr,
                          "[kp] search W=%d todo=%u seeds=%u waves=%u (%d blocks/CU) ring=%u back=%u frames=%llu "
@@ -3002,7 +3306,56 @@ int Engine::finish_epoch(const SearchOpts &o, EpochStats &es)
         int rc = carry_frames(*this, C, size_t(FrameWords(kClassW[c])) * 8, D, ns, o.seed_budget, es);
         if (rc && !first_err) first_err = rc;
     }
+    if (!first_err) {
+        // the one-word class ran this epoch through (or had nothing to run) and
+        // holds no pending frame: no frame names an older derived root
+        bool c0_clean = cls[0].nseed == 0 && cls[0].spill_n == 0;
+        for (auto &cg : got)
+            if (cg.first == 0 && cg.second.pause) c0_clean = false;
+        int rc = take_derived(es, c0_clean);
+        if (rc) first_err = rc;
+    }
+    nar_cur = -1;
     return first_err;
+}
+
+// Hands the derived roots of the epoch that ended (narrowed frames of wide
+// classes) to the one-word class: their frames join its resumed frames, on the
+// device; their roots stay in the buffer, which stays in use until no frame
+// can name them.
+int Engine::take_derived(EpochStats &es, bool c0_clean)
+{
+    if (c0_clean)
+        for (int b = 0; b < int(NDERIVED_BUF); b++)
+            if (b != nar_cur) nar[b].in_use = false;
+    if (nar_cur < 0) return KP_OK;
+    unsigned long long ctr[2] = {0, 0};
+    HIPCHK(hipMemcpy(ctr, nar_ctr.p, sizeof(ctr), hipMemcpyDeviceToHost));
+    const uint32_t nd = uint32_t(std::min<unsigned long long>(ctr[1], nar_cap));
+    NarBuf &B = nar[nar_cur];
+    if (std::getenv("KP_STATS"))
+        std::fprintf(stderr, "[kp] narrowed %u frames into the one-word class (buffer %d, arena %.1f of %.1f MiB)\n", nd,
+                     nar_cur, double(std::min<unsigned long long>(ctr[0], B.arena.bytes / 8)) * 8 / 1048576.0,
+                     double(B.arena.bytes) / 1048576.0);
+    if (!nd) return KP_OK;
+    B.in_use = true;
+    ClassState &C0 = cls[0];
+    const size_t fb = size_t(FrameWords(1)) * 8;
+    DevBuf &nb = C0.seed[1 - C0.cur_seed];
+    int rc;
+    if ((rc = nb.ensure(size_t(C0.nseed + nd) * fb))) return rc;
+    if (C0.nseed)
+        HIPCHK(hipMemcpyAsync(nb.p, C0.seed[C0.cur_seed].p, size_t(C0.nseed) * fb, hipMemcpyDeviceToDevice, side));
+    HIPCHK(hipMemcpyAsync(static_cast<char *>(nb.p) + size_t(C0.nseed) * fb, B.frames.p, size_t(nd) * fb,
+                          hipMemcpyDeviceToDevice, side));
+    HIPCHK(hipStreamSynchronize(side));
+    C0.cur_seed = 1 - C0.cur_seed;
+    C0.nseed += nd;
+    C0.maxn = std::max<uint32_t>(C0.maxn, 64);
+    C0.blocks_per_cu = 0;   // a fresh grid, not the one it last ran on
+    total_narrowed += nd;
+    es.narrowed += nd;
+    return KP_OK;
 }
 
 } // namespace kp
@@ -3085,6 +3438,14 @@ SearchOpts search_opts(const Engine &E, const kp_params &P, int device, int &rc)
     const double t = (P.thres > 0 && P.thres <= 1) ? double(P.thres) : 1.0;
     const uint64_t cap = P.queue_capacity ? P.queue_capacity : kDefaultQueueCap;
     o.seed_budget = std::max<uint64_t>(1, uint64_t(t * double(cap)));
+    o.narrow_from = kNarrowFrom;
+    o.narrow_min_slack = kNarrowMinSlack;
+    if (const char *e = std::getenv("KP_NARROW_FROM")) o.narrow_from = std::atoi(e);
+    if (const char *e = std::getenv("KP_NARROW_SLACK")) o.narrow_min_slack = uint32_t(std::atoi(e));
+    o.regrid = !(std::getenv("KP_REGRID") && std::atoi(std::getenv("KP_REGRID")) == 0);
+    o.narrow_pause = true;
+    o.pausable = o.pausable || o.regrid;
+    if (P.flags & KP_FLAG_NO_NARROW) o.narrow_from = 0;
     (void)E;
     return o;
 }
@@ -3187,9 +3548,10 @@ int run_pool(Engine &E, const std::vector<JobIn> &jobs, const kp_params &P0, std
     if ((rc = E.list_used.ensure(8))) return rc;
     HIPCHK(hipMemsetAsync(E.list_used.p, 0, 8, E.stream));
     EpochStats es;
-    const uint64_t spilled0 = E.total_spilled, pauses0 = E.total_pauses;
+    const uint64_t spilled0 = E.total_spilled, pauses0 = E.total_pauses, narrowed0 = E.total_narrowed;
     while (E.has_work()) {
         if ((rc = E.launch_epoch(o))) return rc;
+        if ((rc = E.watch_epoch(o))) return rc;
         if ((rc = E.finish_epoch(o, es))) return rc;
     }
     HIPCHK(hipEventRecord(e2, E.stream));
@@ -3218,6 +3580,7 @@ int run_pool(Engine &E, const std::vector<JobIn> &jobs, const kp_params &P0, std
         r->pm = jobs[i].g.pm;
         r->spilled = E.total_spilled - spilled0;
         r->pauses = E.total_pauses - pauses0;
+        r->narrowed = E.total_narrowed - narrowed0;
     }
     if (o.want_list) {
         uint64_t used = 0;

@@ -21,6 +21,12 @@
 namespace kp {
 
 constexpr int kNumClass = 7;
+// narrowing defaults: classes of >= 2 words, frames with |P| + |C ∪ D| >= q + 10;
+// buffers of a 4 GiB arena and 4M derived roots
+constexpr int kNarrowFrom = 2;
+constexpr uint32_t kNarrowMinSlack = 10;
+constexpr size_t kNarrowArenaBytes = size_t(4) << 30;
+constexpr uint32_t kNarrowRoots = 1u << 22;
 extern const int kClassW[kNumClass];   // bitset words per size class
 int class_of(uint32_t n);              // -1 when n > 64 * 16
 
@@ -87,12 +93,22 @@ struct SearchOpts {
     bool host_pause = false;      // waves poll the engine's pinned pause word
     bool pausable = false;        // allocate dump arrays (needed by either pause source)
     uint64_t seed_budget = 1;     // resumed frames kept on the device per class
+    // narrowing: frames of classes of >= narrow_from words whose live set fits
+    // one word become derived roots of the one-word class (0 = off)
+    int narrow_from = 0;
+    uint32_t narrow_min_slack = 0;   // ... when |P| + |C ∪ D| - q is at least this
+    bool narrow_pause = false;       // a wide class whose buffer is full pauses (needs pausable)
+    // once only classes launched on a reduced grid are still running, pause
+    // them so that the next epoch gives them the whole chip
+    bool regrid = false;
 };
 
 struct ClassState {
     DevBuf q, qseq, sched, stack, subq, desc, todo, seed[2], dump;
     hipStream_t stream = nullptr;
-    int blocks_per_cu = 0;
+    int blocks_per_cu = 0, nb_max = 0;  // blocks per CU of the last launch / the occupancy limit
+    bool regrid_sent = false;           // the host asked the running launch to pause for a larger grid
+    bool demoted = false;               // the last launch ran at one block per CU beside a primary class
     std::vector<kpd::RootDesc> hdesc;   // root table (frames name roots by index)
     std::vector<uint32_t> htodo;        // start list
     size_t todo_done = 0;               // entries of htodo already started
@@ -113,7 +129,7 @@ struct ClassState {
 };
 
 struct EpochStats {
-    uint64_t frames = 0, donated = 0, spilled = 0, refilled = 0;
+    uint64_t frames = 0, donated = 0, spilled = 0, refilled = 0, narrowed = 0;
     bool paused = false;
 };
 
@@ -133,6 +149,19 @@ struct Engine {
     ClassState cls[kNumClass];
     uint32_t *hpause = nullptr, *dpause = nullptr;   // pinned host pause word
     uint64_t total_spilled = 0, total_pauses = 0;
+    // Derived-root buffers of narrowed frames.  An epoch fills one; its roots
+    // stay in use until the one-word class has run an epoch through without a
+    // pause and holds no pending frame (then every older buffer is free).
+    struct NarBuf {
+        DevBuf arena, desc, frames;
+        bool in_use = false;
+    };
+    NarBuf nar[kpd::NDERIVED_BUF];
+    DevBuf nar_ctr;
+    int nar_cur = -1;                  // buffer filled by the running epoch (-1 = narrowing off)
+    uint32_t nar_cap = 0;              // derived roots per buffer
+    uint64_t total_narrowed = 0;
+    int take_derived(EpochStats &es, bool c0_clean);
 
     int init();
     ~Engine();
@@ -169,6 +198,9 @@ struct Engine {
     // allocates the pinned, device-mapped pause word that SearchOpts::host_pause polls
     int enable_host_pause();
     bool epoch_done();                 // every launched class has finished (does not block)
+    // waits for the running epoch; once every class still running was launched
+    // on a reduced grid (and another has finished), pauses them (regrid)
+    int watch_epoch(const SearchOpts &o);
     // launches class c alone over its whole start list and resumed frames and
     // returns the kernel time; the class's tables are left as they were
     int time_class(int c, const SearchOpts &o, float *ms);

@@ -51,6 +51,7 @@ struct Sched {             // persistent-kernel scheduler state (one per launch)
     uint32_t dump_used;    // frames written to the dump array
     int32_t error;
     uint32_t retired;      // waves that exited early as surplus
+    uint32_t started;      // waves that have become resident
     uint64_t epoch_frames; // frames processed, in steps of the check interval
     uint64_t list_unused;
     uint64_t donated;
@@ -101,6 +102,30 @@ struct BuildArgs {
     uint64_t *wr_extra;   // per desc: sum of degrees over B and L
 };
 
+// Narrowing: a frame of a wide size class whose live vertices (P, C, the
+// k-search candidates and the excluded vertices that can still extend a plex
+// of its subtree) fit one word is re-rooted as a derived root of the one-word
+// class.  The derived subgraph keeps those vertices in block order, so the
+// one-word search walks the same tree.  An epoch fills one derived-root buffer
+// (subgraph arena, root table, one frame per root); the one-word class resumes
+// those frames in the next epoch and reads their roots from that buffer.  A
+// frame names a derived root as DERIVED_BIT | buffer << DERIVED_SHIFT | index.
+constexpr uint32_t DERIVED_BIT = 0x80000000u;
+constexpr uint32_t DERIVED_SHIFT = 28;
+constexpr uint32_t NDERIVED_BUF = 4;
+constexpr uint32_t DERIVED_INDEX_MASK = (1u << DERIVED_SHIFT) - 1u;
+struct NarrowArgs {
+    uint64_t *arena;              // derived subgraphs, bump-allocated
+    uint64_t arena_cap;           // words
+    unsigned long long *ctr;      // [0] arena words reserved, [1] derived roots reserved
+    RootDesc *desc;               // derived root i
+    uint64_t *frames;             // its frame (one-word layout, FrameWords(1) words)
+    uint32_t cap;                 // derived roots (0 = narrowing off)
+    uint32_t tag;                 // DERIVED_BIT | buffer << DERIVED_SHIFT
+    uint32_t min_slack;           // only frames with |P| + |C ∪ D| >= q + min_slack are narrowed
+    uint32_t pause_when_full;     // a wave that finds the buffer full pauses its class
+};
+
 struct SearchArgs {
     const RootDesc *desc;      // class root table
     const uint32_t *todo;      // roots to start, in order (indices into desc)
@@ -125,10 +150,13 @@ struct SearchArgs {
     uint32_t donate_period, donate_target;
     uint64_t limit_ticks;      // 0 = no limit; else wall-clock ticks per launch
     const uint32_t *host_pause;   // mapped host word (nonzero = pause) or null
+    const uint32_t *class_pause;  // mapped host word of this class alone, or null
     uint64_t pause_frames;     // 0 = none; else pause once this many frames were processed
     unsigned long long *t0;    // shared start tick of the epoch (0 until the first wave of any class starts)
     uint32_t retire_margin;    // hungry waves kept beyond the busy ones once nothing is left to start
     uint64_t retire_ticks;     // wall-clock ticks a wave stays hungry before it may retire
+    NarrowArgs nar;            // wide classes only
+    const RootDesc *ddesc[NDERIVED_BUF];   // one-word class: derived root tables
 };
 
 struct RootKernArgs {

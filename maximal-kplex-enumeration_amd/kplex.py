@@ -30,6 +30,7 @@ KP_FLAG_LIST = 1
 KP_FLAG_VALIDATE = 2
 KP_FLAG_VERBOSE = 4
 KP_FLAG_GPU_PREP = 8
+KP_FLAG_NO_NARROW = 16
 
 
 class KpGraph(ctypes.Structure):
@@ -50,7 +51,7 @@ class KpResult(ctypes.Structure):
                 ("preprocess_ms", c_double), ("upload_ms", c_double), ("decompose_ms", c_double),
                 ("search_ms", c_double), ("wall_ms", c_double), ("spilled", c_uint64), ("pauses", c_uint64),
                 ("nlisted", c_uint64), ("list_off", POINTER(c_uint64)), ("list_ids", POINTER(c_uint32)),
-                ("pn_first", c_uint32), ("status", ctypes.c_int32)]
+                ("pn_first", c_uint32), ("status", ctypes.c_int32), ("narrowed", c_uint64)]
 
 
 class KpTune(ctypes.Structure):
@@ -263,15 +264,16 @@ def make_params(k, q, thres=0.25, truss=0, flags=0, time_limit_s=0.0, queue_capa
 
 
 def enumerate_plexes(graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0, validate=False, roots=None,
-                     queue_capacity=0, pause_every=0, gpu_prep=False):
+                     queue_capacity=0, pause_every=0, gpu_prep=False, narrow=True):
     """kp_enumerate: returns a dict with count, hash, tasks, timings (and plexes when listing).
     roots=(b, e) restricts the enumeration to roots at degeneracy positions [b, e).
     pause_every > 0 pauses each search launch after about that many frames and
     resumes it; thres * queue_capacity frames stay on the device, the rest spill
-    to pinned host memory."""
+    to pinned host memory.  narrow=False keeps every frame in its root's size
+    class (results are the same either way)."""
     lib = load()
     flags = (KP_FLAG_LIST if listing else 0) | (KP_FLAG_VALIDATE if validate else 0) | \
-        (KP_FLAG_GPU_PREP if gpu_prep else 0)
+        (KP_FLAG_GPU_PREP if gpu_prep else 0) | (0 if narrow else KP_FLAG_NO_NARROW)
     p = make_params(k, q, thres, truss, flags, time_limit_s, queue_capacity, roots=roots, pause_every=pause_every)
     r = KpResult()
     _check(lib.kp_enumerate(ctypes.byref(graph.g), ctypes.byref(p), ctypes.byref(r)), "kp_enumerate")
@@ -311,12 +313,12 @@ class Prepared:
     """kp_prepare / kp_run: the graph preprocessed and resident in HBM."""
 
     def __init__(self, graph, k, q, thres=0.25, truss=0, listing=False, time_limit_s=0.0, roots=None,
-                 queue_capacity=0, pause_every=0):
+                 queue_capacity=0, pause_every=0, narrow=True):
         self.lib = load()
         self.h = c_void_p()
         self.listing = listing
-        p = make_params(k, q, thres, truss, KP_FLAG_LIST if listing else 0, time_limit_s, queue_capacity,
-                        roots=roots, pause_every=pause_every)
+        p = make_params(k, q, thres, truss, (KP_FLAG_LIST if listing else 0) | (0 if narrow else KP_FLAG_NO_NARROW),
+                        time_limit_s, queue_capacity, roots=roots, pause_every=pause_every)
         _check(self.lib.kp_prepare(ctypes.byref(graph.g), ctypes.byref(p), ctypes.byref(self.h)), "kp_prepare")
 
     def run(self):
