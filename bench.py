@@ -296,18 +296,20 @@ def small_line(reps=7):
             "gpu_vs_cpu": c_med / g_med, "identical": True}
 
 
-def full_graph_line(time_limit):
-    """The Chung-Lu 75k workload in full: every root position, enumerated as
-    consecutive windows (windows partition the result, so counts, task counts
-    and hashes add up), each window one kp_run that prints its line when it
-    ends; the last, heaviest positions go in narrower windows.  The sum is the
-    whole graph's exact count."""
+def full_graph_line(time_limit, lo=0, hi=None):
+    """The Chung-Lu 75k workload in full: every root position in [lo, hi)
+    (default: all), enumerated as consecutive windows (windows partition the
+    result, so counts, task counts and hashes add up), each window one kp_run
+    that prints its line when it ends; the last, heaviest positions go in
+    narrower windows.  Over [0, end) the sum is the whole graph's exact count."""
     import kplex
     desc, fn, _ = WORKLOADS["cl75k-full"]
     g, k, q, thres, tr = getattr(kplex, fn)()
     end = len(kplex.preprocess(g, k, q, tr)[0]) - q + 2
-    cuts = [0, 9000] + list(range(9500, min(end, 16000), 500)) + list(range(16000, end, 50)) + [end]
-    cuts = sorted(set(c for c in cuts if c <= end))
+    hi = end if hi is None else min(hi, end)
+    cuts = ([0, 9000] + list(range(9500, min(end, 16000), 500)) + list(range(16000, min(end, 16400), 100)) +
+            list(range(16400, end, 25)) + [end, lo, hi])
+    cuts = sorted(set(c for c in cuts if lo <= c <= hi))
     windows = list(zip(cuts[:-1], cuts[1:]))
     tot = {"count": 0, "tasks": 0, "hash_a": 0, "hash_b": 0, "search_ms": 0.0, "valid_roots": 0}
     per = []
@@ -327,7 +329,7 @@ def full_graph_line(time_limit):
             f"(total {tot['count']} after {time.perf_counter() - t0:.1f} s)")
     wall = time.perf_counter() - t0
     p.close()
-    return {"workload": desc, "roots": end, "windows": len(windows), "count": tot["count"],
+    return {"workload": desc, "roots": end, "positions": [lo, hi], "windows": len(windows), "count": tot["count"],
             "hash": f"{tot['hash_b']:016x}{tot['hash_a']:016x}", "tasks": tot["tasks"],
             "valid_roots": tot["valid_roots"], "gpu_s": tot["search_ms"] / 1e3, "wall_s": wall,
             "plexes_per_s": tot["count"] / wall, "per_window": per}
@@ -376,14 +378,17 @@ def main():
     ap.add_argument("--full-graph", action="store_true",
                     help="also enumerate the whole Chung-Lu 75k graph (every root; many minutes of GPU time)")
     ap.add_argument("--full-graph-only", action="store_true", help="only the whole-graph line, printed as JSON")
+    ap.add_argument("--full-graph-range", default=None, metavar="LO:HI",
+                    help="root positions of the whole-graph line (default: all)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device (counter calibration for the traffic figure) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
+    fg_range = tuple(int(x) for x in args.full_graph_range.split(":")) if args.full_graph_range else (0, None)
     if args.full_graph_only:
-        print(json.dumps({"full_graph": full_graph_line(args.time_limit)}))
+        print(json.dumps({"full_graph": full_graph_line(args.time_limit, *fg_range)}))
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -472,7 +477,7 @@ def main():
     if args.served_seconds > 0:
         out["served"] = served_vs_plain(args.served_seconds, args.served_rate, args.time_limit)
     if args.full_graph:
-        out["full_graph"] = full_graph_line(args.time_limit)
+        out["full_graph"] = full_graph_line(args.time_limit, *fg_range)
     # the other parts of the metric, at the top level
     out["wall_s_at_exact_count"] = out["wall_s_per_step"]
     out["batched_vs_plain"] = out["batched"]["batched_vs_plain"] if "batched" in out else None

@@ -43,6 +43,8 @@
 
 namespace kpd {
 
+constexpr uint32_t kSeedChunk = 16;   // resumed frames a wave claims at once, at most
+
 constexpr uint32_t NCAP = KP_MAX_BLOCK;          // largest block
 constexpr int WPB = 4;                           // waves per block (256 threads)
 constexpr uint32_t EXCL = 0x80000000u;           // counter-array flags
@@ -575,7 +577,7 @@ struct Worker {
     // registers of the frame loop free of state it touches once per plex)
     uint32_t tk = 0;
     int err = 0;
-    uint32_t nar_full = 0;   // set once a derived-root reservation failed
+    uint32_t nar_on = 0;     // narrowing: 0 not yet (epoch younger than nar.after_ticks), 1 on, 2 buffer full
 
     __device__ Worker(const SearchArgs &args, uint64_t *ring_, uint64_t *back_, Cold *cold_, uint32_t *sel_ = nullptr)
         : a(args), S(args.sched), ring(ring_), back(back_), cold(cold_), sel(sel_)
@@ -635,7 +637,7 @@ struct Worker {
         }
         tk = rfl(tk);
         err = int(rfl(uint32_t(err)));
-        nar_full = rfl(nar_full);
+        nar_on = rfl(nar_on);
     }
     static __device__ __forceinline__ void launder(Frame<W> &f)
     {
@@ -681,9 +683,20 @@ struct Worker {
     // lane's store is harmless.  Wider slots go through lane 0.  Loaded words
     // are broadcast with readlane / readfirstlane.
     static constexpr bool kLaneSlots = FW <= 64 && W <= 4;
+    // Shared: a queue slot that another wave reads once its sequence number
+    // is published.  Its words are written with agent-scope stores, which on
+    // MI355X write through the XCD's L2, so the reader (another XCD) finds
+    // them in memory once the stores are acknowledged; visibility then does
+    // not rest on the release fence's L2 write-back completing before the
+    // sequence-number store (the compiler emits no wait after the write-back).
+    template <bool Shared = false>
     __device__ __forceinline__ void store_slot(uint64_t *dst, uint64_t meta, const uint64_t *P, const uint64_t *C,
                                                const uint64_t *X, const uint64_t *D, bool withD)
     {
+        auto put = [](uint64_t *p, uint64_t v) {
+            if constexpr (Shared) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else *p = v;
+        };
         if constexpr (kLaneSlots) {
             uint64_t v = meta;
 #pragma unroll
@@ -693,16 +706,16 @@ struct Worker {
                 v = inmask(1ull << (1 + 2 * W + w)) ? X[w] : v;
                 if (withD) v = inmask(1ull << (1 + 3 * W + w)) ? D[w] : v;
             }
-            dst[min(ln, uint32_t(FW - 1))] = v;
+            put(dst + min(ln, uint32_t(FW - 1)), v);
         } else {
             if (lane0()) {
-                dst[0] = meta;
+                put(dst, meta);
 #pragma unroll
                 for (int w = 0; w < W; w++) {
-                    dst[1 + w] = P[w];
-                    dst[1 + W + w] = C[w];
-                    dst[1 + 2 * W + w] = X[w];
-                    if (withD) dst[1 + 3 * W + w] = D[w];
+                    put(dst + 1 + w, P[w]);
+                    put(dst + 1 + W + w, C[w]);
+                    put(dst + 1 + 2 * W + w, X[w]);
+                    if (withD) put(dst + 1 + 3 * W + w, D[w]);
                 }
             }
             asm volatile("");
@@ -776,10 +789,11 @@ struct Worker {
     }
     __device__ __forceinline__ uint32_t depth() const { return rn + bn; }
 
-    __device__ bool push(uint64_t meta, const uint64_t *P, const uint64_t *C, const uint64_t *X, const uint64_t *D,
-                         bool withD)
+    // a free ring slot on top: a full ring moves its oldest frame to the HBM
+    // backing stack
+    __device__ __forceinline__ bool make_room()
     {
-        if (rn == a.ring_cap) {   // spill the oldest ring frame to the HBM backing stack
+        if (rn == a.ring_cap) {
             if (bn == a.back_cap) {
                 set_error(KP_EOVERFLOW);
                 return false;
@@ -790,9 +804,27 @@ struct Worker {
             rn--;
             cold->spills = rfl(cold->spills) + 1;
         }
+        return true;
+    }
+    __device__ bool push(uint64_t meta, const uint64_t *P, const uint64_t *C, const uint64_t *X, const uint64_t *D,
+                         bool withD)
+    {
+        if (!make_room()) return false;
         store_slot(ring_slot(rn), meta, P, C, X, D, withD);
         rn++;
         return true;
+    }
+    // pushes a copy of a frame slot (a resumed frame)
+    __device__ bool push_copy(const uint64_t *src)
+    {
+        if (!make_room()) return false;
+        copy_slot(ring_slot(rn), src);
+        rn++;
+        return true;
+    }
+    __device__ __forceinline__ const uint64_t *seed_slot(uint32_t si) const
+    {
+        return si < a.nseed_dev ? a.seed + size_t(si) * FW : a.hseed + size_t(si - a.nseed_dev) * FW;
     }
     __device__ __forceinline__ bool push(const Frame<W> &f)
     {
@@ -867,7 +899,7 @@ struct Worker {
     }
     __device__ void q_publish(size_t slot, uint32_t shard, uint64_t t, const Frame<W> &f)
     {
-        store_slot(a.qframes + slot * FW, make_meta(f.root, f.type, f.r), f.P, f.C, f.X, f.D, f.type == FT_KSEARCH);
+        store_slot<true>(a.qframes + slot * FW, make_meta(f.root, f.type, f.r), f.P, f.C, f.X, f.D, f.type == FT_KSEARCH);
         // One agent-scope release (an L2 write-back on MI355X, whose L2 is
         // per XCD) orders the frame before the slot's sequence number; the
         // counters are relaxed.  A consumer validates the slot with an
@@ -1512,7 +1544,7 @@ struct Worker {
             // no more narrowing in this launch (the next reservations would
             // fail too); the class pauses if asked to, so that the next epoch
             // resumes it with an empty buffer
-            nar_full = 1;
+            nar_on = 2;
             if (N.pause_when_full && lane0()) __hip_atomic_store(&S->pause, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }
@@ -1719,9 +1751,18 @@ struct Worker {
     // the dump as well.  Waves without work count themselves in S->hungry;
     // busy waves donate their oldest frames (the largest pending subtrees)
     // while hungry waves outnumber the queued frames.
+    // Narrowing starts once the epoch has run for nar.after_ticks: a short
+    // epoch gains nothing from it (its derived roots would wait for the next
+    // epoch), a long one (blocks of heavy hub roots) gains much.
+    __device__ __forceinline__ uint32_t narrow_state() const
+    {
+        if (nar_on != 0 || a.nar.cap == 0) return nar_on;
+        return wall_clock64() - rfl64(cold->t_start) >= a.nar.after_ticks ? 1u : 0u;
+    }
     __device__ void run()
     {
         if (lane0()) atomicAdd(&S->started, 1u);
+        if constexpr (W >= 2) nar_on = narrow_state();
         bool active_self = false, hungry_self = false, have = false, paused = false;
         uint32_t to_donate = a.donate_period, backoff = 1;   // frames left until the next donation check
         uint32_t to_check = rfl(cold->check_every);   // frames left until the next check
@@ -1769,20 +1810,29 @@ struct Worker {
                         if (err) break;
                     }
                     if (!got && ns < a.nseed) {
+                        // Resumed frames are claimed a few at a time (every
+                        // wave shares the one cursor): the first is processed
+                        // now, the others go on this wave's stack, from which
+                        // donations can still move them.
+                        const uint32_t want = max(1u, min(a.seed_chunk, (a.nseed - ns) / (2u * a.nwaves)));
                         uint32_t si = 0xffffffffu;
                         if (lane0()) {
                             __hip_atomic_fetch_add(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-                            si = __hip_atomic_fetch_add(&S->next_seed, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+                            si = __hip_atomic_fetch_add(&S->next_seed, want, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                             if (si >= a.nseed)
                                 __hip_atomic_fetch_sub(&S->active, 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
                         }
                         si = rfl(si);
                         if (si < a.nseed) {
                             active_self = true;
-                            load_slot(si < a.nseed_dev ? a.seed + size_t(si) * FW : a.hseed + size_t(si - a.nseed_dev) * FW, f);
+                            const uint32_t cnt = min(want, a.nseed - si);
+                            for (uint32_t j = cnt; j-- > 1;)
+                                if (!push_copy(seed_slot(si + j))) break;
+                            load_slot(seed_slot(si), f);
                             have = true;
                             got = true;
                         }
+                        if (err) break;
                     }
                     if (!got && nr < a.ntodo) {
                         uint32_t rr = 0xffffffffu;
@@ -1894,7 +1944,7 @@ struct Worker {
                 }
                 Frame<W> nf;
                 bool narrowed = false;
-                if constexpr (W >= 2) narrowed = a.nar.cap != 0 && !nar_full && try_narrow(f);
+                if constexpr (W >= 2) narrowed = nar_on == 1 && try_narrow(f);
                 if (narrowed) have = false;
                 else have = (f.type == FT_KSEARCH) ? ksearch(f, nf) : branch(f, nf);
                 nfr++;
@@ -1939,6 +1989,7 @@ struct Worker {
                 to_check = ce;
                 fold();
                 cold->frames = cold->frames + ce;
+                if constexpr (W >= 2) nar_on = narrow_state();
                 uint32_t stop = 0;
                 if (lane0()) {
                     stop = __hip_atomic_load(&S->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
@@ -2657,7 +2708,9 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary, bool alo
     ClassState &C = E.cls[c];
     constexpr int FW = FrameLayout<W>::FW;
     const ClassTune &T = g_tune[c];
-    const uint32_t need = C.maxn + 3;   // DFS depth bound: every child shrinks C2 or C
+    // DFS depth bound (every child shrinks C2 or C), plus the resumed frames a
+    // wave claims at once below its first
+    const uint32_t need = C.maxn + 3 + kSeedChunk;
     // LDS ring: 8 KiB of frames per wave unless tuned; at most 64 KiB per block
     const uint32_t ring_max = std::max<uint32_t>(4, (64 * 1024) / (FW * 8 * 4));
     uint32_t ring = T.ring_frames ? std::max<uint32_t>(4, T.ring_frames) : std::max<uint32_t>(16, 8192 / (FW * 8));
@@ -2709,6 +2762,8 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary, bool alo
     const uint32_t nwg = uint32_t(nb) * uint32_t(E.cus);
     SearchArgs sa{};
     sa.nwaves = nwg * 4;
+    sa.seed_chunk = kSeedChunk;
+    if (const char *e = std::getenv("KP_SEED_CHUNK")) sa.seed_chunk = std::min<uint32_t>(kSeedChunk, std::max(1, std::atoi(e)));
     sa.qcap = T.queue_slots ? std::max<uint32_t>(16, T.queue_slots) : std::max<uint32_t>(256, sa.nwaves * 4 / NSUBQ);
     sa.ring_cap = ring;
     sa.back_cap = back;
@@ -2781,6 +2836,7 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary, bool alo
         sa.nar.tag = DERIVED_BIT | (uint32_t(E.nar_cur) << DERIVED_SHIFT);
         sa.nar.min_slack = o.narrow_min_slack;
         sa.nar.pause_when_full = o.narrow_pause && o.pausable;
+        sa.nar.after_ticks = uint64_t(o.narrow_after_ms * double(E.ticks_per_ms));
     }
     if (W == 1)
         for (uint32_t b = 0; b < NDERIVED_BUF; b++) sa.ddesc[b] = E.nar[b].desc.as<RootDesc>();
@@ -3157,6 +3213,7 @@ int Engine::enable_host_pause()
 int Engine::watch_epoch(const SearchOpts &o)
 {
     if (!o.regrid || !hpause) return KP_OK;
+    const double t_launch = now_ms();
     for (;;) {
         bool running = false, finished = false, blocker = false;
         int cand[kNumClass], nc = 0;
@@ -3177,8 +3234,10 @@ int Engine::watch_epoch(const SearchOpts &o)
         if (!running) return KP_OK;
         // Every class still running was demoted to one block per CU beside a
         // primary class that has drained: pause them, and the next epoch
-        // launches the one with the most pending work on the whole chip.
-        if (finished && !blocker && nc) {
+        // launches the one with the most pending work on the whole chip.  Not
+        // in a young epoch: a pause and relaunch cost more than what a short
+        // remainder gains (email-euall: 184 -> 207 ms when always on).
+        if (finished && !blocker && nc && now_ms() - t_launch >= o.regrid_after_ms) {
             for (int i = 0; i < nc; i++) {
                 __atomic_store_n(&hpause[1 + cand[i]], 1u, __ATOMIC_RELEASE);
                 cls[cand[i]].regrid_sent = true;
@@ -3444,6 +3503,10 @@ SearchOpts search_opts(const Engine &E, const kp_params &P, int device, int &rc)
     if (const char *e = std::getenv("KP_NARROW_SLACK")) o.narrow_min_slack = uint32_t(std::atoi(e));
     o.regrid = !(std::getenv("KP_REGRID") && std::atoi(std::getenv("KP_REGRID")) == 0);
     o.narrow_pause = true;
+    o.narrow_after_ms = kNarrowAfterMs;
+    o.regrid_after_ms = kRegridAfterMs;
+    if (const char *e = std::getenv("KP_NARROW_AFTER_MS")) o.narrow_after_ms = std::atof(e);
+    if (const char *e = std::getenv("KP_REGRID_AFTER_MS")) o.regrid_after_ms = std::atof(e);
     o.pausable = o.pausable || o.regrid;
     if (P.flags & KP_FLAG_NO_NARROW) o.narrow_from = 0;
     (void)E;

@@ -27,6 +27,10 @@ constexpr int kNarrowFrom = 2;
 constexpr uint32_t kNarrowMinSlack = 10;
 constexpr size_t kNarrowArenaBytes = size_t(4) << 30;
 constexpr uint32_t kNarrowRoots = 1u << 22;
+// narrowing and regrid pay only in long epochs (measured: soc-epinions 184 ->
+// 310 ms and email-euall 184 -> 348 ms with narrowing from the first frame)
+constexpr double kNarrowAfterMs = 250.0;
+constexpr double kRegridAfterMs = 250.0;
 extern const int kClassW[kNumClass];   // bitset words per size class
 int class_of(uint32_t n);              // -1 when n > 64 * 16
 
@@ -98,6 +102,8 @@ struct SearchOpts {
     int narrow_from = 0;
     uint32_t narrow_min_slack = 0;   // ... when |P| + |C ∪ D| - q is at least this
     bool narrow_pause = false;       // a wide class whose buffer is full pauses (needs pausable)
+    double narrow_after_ms = 0;      // narrowing starts this long into an epoch
+    double regrid_after_ms = 0;      // regrid pauses only in epochs older than this
     // once only classes launched on a reduced grid are still running, pause
     // them so that the next epoch gives them the whole chip
     bool regrid = false;

@@ -124,6 +124,7 @@ struct NarrowArgs {
     uint32_t tag;                 // DERIVED_BIT | buffer << DERIVED_SHIFT
     uint32_t min_slack;           // only frames with |P| + |C ∪ D| >= q + min_slack are narrowed
     uint32_t pause_when_full;     // a wave that finds the buffer full pauses its class
+    uint64_t after_ticks;         // narrowing starts this long after the epoch's first wave
 };
 
 struct SearchArgs {
@@ -133,6 +134,7 @@ struct SearchArgs {
     const uint64_t *seed;      // frames to resume: seed[0, nseed_dev) on the device,
     const uint64_t *hseed;     // then hseed[0, nseed - nseed_dev) in mapped pinned host memory
     uint32_t nseed, nseed_dev;
+    uint32_t seed_chunk;       // resumed frames a wave claims at once (<= the stack room for them)
     uint64_t *dump;            // frames written on pause
     uint32_t dump_cap;
     JobAcc *acc;               // per-job accumulators

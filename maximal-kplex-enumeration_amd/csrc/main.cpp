@@ -1,7 +1,7 @@
 // main.cpp -- command-line front end with the reference's interface
 // (main.cu:11-62):  PlexEnum <dataset> -k <k> -q <q> -t <thres> -tr <truss>
 // It prints the reference's lines (main.cu:30,54; inc/graph.h:47;
-// inc/host_funcs.h:115,1149,1156-1158), with a 64-bit plex count and one
+// inc/host_funcs.h:115,871,873,1149,1156-1158), with a 64-bit plex count and one
 // extra "hash:" line at the end.
 #include "../../include/kplex.h"
 
@@ -71,6 +71,9 @@ int main(int argc, char *argv[])
     // truss on again after the second core (inc/host_funcs.h:115, :749, :823)
     std::printf("\npn = %u\n", r.pn_first);
     if (p.truss) std::printf("\npn = %u\n", r.pn);
+    // the reference's lines around its host->device copy (inc/host_funcs.h:871,873)
+    std::printf("Start copying graph to GPU....\n");
+    std::printf("Done copying graph to GPU....\n");
     std::printf("Total Time Elapsed: %f ms\n", r.wall_ms);
     std::printf("Total Valid Blocks: %llu, Maximal k-Plexes: %llu\n", (unsigned long long)r.valid_roots,
                 (unsigned long long)r.count);

@@ -6,6 +6,7 @@ has the time; the GPU box only reads the JSON).
     python tests/golden/make_golden.py --batch 48        # oracle_batch.json
     python tests/golden/make_golden.py --served          # oracle_served.json
     python tests/golden/make_golden.py --cl75k-chunks    # oracle_cl75k_chunks.json
+    python tests/golden/make_golden.py --cl75k-hubs      # oracle_cl75k_hubs.json
 
 Each entry records count, 128-bit hash, canonical task count, valid roots,
 preprocessed sizes and the oracle's wall time on this container's cores.
@@ -124,6 +125,22 @@ def cl75k_chunks(threads, step):
         b = e
 
 
+def cl75k_hubs(threads):
+    """Hub roots of the Chung-Lu 75k workload: degeneracy positions [16400,16401)
+    (one root, a 267-vertex block in the six-word class) and [16401,16405)."""
+    path = os.path.join(ROOT, "tests", "golden", "oracle_cl75k_hubs.json")
+    gold = load_json(path)
+    items = gold.setdefault("items", {})
+    g, k, q, _, tr = kplex.workload_cl75k()
+    og = from_graph(g)
+    for b, e in ((16400, 16401), (16401, 16405)):
+        key = f"{b}_{e}"
+        if key not in items:
+            items[key] = per_item(og, k, q, tr, window=(b, e), threads=threads)
+            save_json(path, gold)
+            print("cl75k hubs", key, items[key], flush=True)
+
+
 def from_graph(g):
     deg, nbr = g.csr()
     return kpo.OGraph.from_csr(deg, nbr)
@@ -138,10 +155,13 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="oracle results of the first N many-graph members")
     ap.add_argument("--served", action="store_true", help="oracle results of the served fixture queries")
     ap.add_argument("--cl75k-chunks", action="store_true", help="Chung-Lu 75k root windows past 9000")
+    ap.add_argument("--cl75k-hubs", action="store_true", help="Chung-Lu 75k hub root windows [16400,16405)")
     ap.add_argument("--chunk", type=int, default=250)
     ap.add_argument("--threads", type=int, default=0)
     args = ap.parse_args()
-    if args.batch or args.served or args.cl75k_chunks:
+    if args.batch or args.served or args.cl75k_chunks or args.cl75k_hubs:
+        if args.cl75k_hubs:
+            cl75k_hubs(args.threads)
         if args.batch:
             batch_fixtures(args.batch, args.threads)
         if args.served:

@@ -182,3 +182,19 @@ def test_autotune_keeps_results():
     finally:
         kplex.set_tune(None)
         p.close()
+
+
+def test_narrowing_hub_roots_equal_the_unnarrowed_run():
+    # Hub roots of the Chung-Lu 75k workload (degeneracy positions [16400,16405):
+    # blocks of 250-270 vertices, four- and six-word classes, 3.1e9 Branch tasks).
+    # Frames whose live set fits 64 vertices become derived roots of the one-word
+    # class; count, hash and the task count (the tree) must not change.
+    g, k, q, _, tr = kplex.workload_cl75k()
+    out = {}
+    for narrow in (True, False):
+        say(f"cl75k hub window [16400,16405) narrow={narrow}")
+        p = kplex.Prepared(g, k, q, truss=tr, time_limit_s=LIMIT, roots=(16400, 16405), narrow=narrow)
+        out[narrow] = p.run()
+        p.close()
+    assert out[True]["narrowed"] > 0 and out[False]["narrowed"] == 0
+    same(out[True], out[False])
