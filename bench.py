@@ -169,7 +169,7 @@ def gen_graphs(specs):
         return list(ex.map(kplex.graph_from_spec, specs))
 
 
-def served_vs_plain(duration_s, rate, time_limit, seed=7):
+def served_vs_plain(duration_s, rate, time_limit, seed=7, slice_ms=30.0, plain=True):
     """The served workload: 256 resident graphs built like the many-graph
     workload but with n up to 200,000; queries (graph uniform, k in {3,4,5},
     q = 5k+10) arrive as a Poisson stream of `rate` per second for
@@ -185,7 +185,7 @@ def served_vs_plain(duration_s, rate, time_limit, seed=7):
     arrivals, queries = kplex.served_queries(rate, duration_s, seed, len(graphs))
     log(f"[bench] served: {len(graphs)} graphs generated in {gen_s:.1f} s; {len(queries)} queries at {rate}/s "
         f"over {duration_s} s")
-    srv = kplex.Server(max_inflight=128, slice_ms=2.0)
+    srv = kplex.Server(max_inflight=128, slice_ms=slice_ms)
     ids = [srv.add_graph(g) for g in graphs]
     t0 = time.perf_counter()
     srv.preload([(ids[i], k, 5 * k + 10) for i in range(len(graphs)) for k in (3, 4, 5)])
@@ -216,7 +216,10 @@ def served_vs_plain(duration_s, rate, time_limit, seed=7):
     lat = sorted(done[i][0] - arrivals[i] for i in range(len(queries)))
     pct = lambda p: lat[min(len(lat) - 1, int(p * len(lat)))]
     count = sum(r["count"] for _, r in done.values())
-    log(f"[bench] served: makespan {makespan:.2f} s, p50 {pct(0.5) * 1e3:.0f} ms, p99 {pct(0.99) * 1e3:.0f} ms")
+    log(f"[bench] served: makespan {makespan:.2f} s, p50 {pct(0.5) * 1e3:.0f} ms, p99 {pct(0.99) * 1e3:.0f} ms "
+        f"(admission slice {slice_ms} ms)")
+    if not plain:
+        return {"queries": len(queries), "makespan_s": makespan, "p50_ms": pct(0.5) * 1e3, "p99_ms": pct(0.99) * 1e3}
     t0 = time.perf_counter()
     for i, (gi, k, q) in enumerate(queries):
         r = kplex.enumerate_plexes(graphs[gi], k, q, time_limit_s=time_limit)
@@ -372,6 +375,8 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=1.5,
                     help="arrival window of the served stream at --served-rate queries/s (0 = skip)")
     ap.add_argument("--served-rate", type=float, default=200.0)
+    ap.add_argument("--served-slice-ms", type=float, default=30.0,
+                    help="age at which a running epoch is paused to admit waiting queries")
     ap.add_argument("--no-autotune", action="store_true", help="time with the built-in launch settings")
     ap.add_argument("--no-spill-line", action="store_true", help="skip the pause/spill line on the 320k workload")
     ap.add_argument("--no-small-line", action="store_true", help="skip the GPU-vs-CPU line on the small planted workload")
@@ -475,7 +480,8 @@ def main():
     if args.batch_graphs > 0:
         out["batched"] = batched_vs_plain(args.batch_graphs, args.time_limit, args.batch_chunks)
     if args.served_seconds > 0:
-        out["served"] = served_vs_plain(args.served_seconds, args.served_rate, args.time_limit)
+        out["served"] = served_vs_plain(args.served_seconds, args.served_rate, args.time_limit,
+                                        slice_ms=args.served_slice_ms)
     if args.full_graph:
         out["full_graph"] = full_graph_line(args.time_limit, *fg_range)
     # the other parts of the metric, at the top level

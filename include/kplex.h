@@ -231,7 +231,7 @@ typedef struct kp_server_config {
                                    (0 = 4 GiB); the search stacks and queues are a fixed
                                    per-size-class cost, not per query */
     double   slice_ms;          /* minimum epoch length before a pause for waiting queries
-                                   (0 = 2 ms) */
+                                   (0 = 30 ms) */
 } kp_server_config;
 int  kp_server_create(const kp_server_config *cfg, kp_server **out);
 /* Registers a graph (copied). */

@@ -432,7 +432,10 @@ extern "C" int kp_server_create(const kp_server_config *cfg, kp_server **out)
     if (cfg) s->cfg = *cfg;
     if (!s->cfg.max_inflight) s->cfg.max_inflight = 64;
     if (!s->cfg.mem_budget_bytes) s->cfg.mem_budget_bytes = 4ull << 30;
-    if (s->cfg.slice_ms <= 0) s->cfg.slice_ms = 2.0;
+    // Each admission pause dumps and relaunches the running classes; on the
+    // served workload (281 queries at 200/s) the makespan was 4.60 s at 2 ms,
+    // 4.33 s at 10 ms and 4.30 s at 30 ms (p50 3.05 / 2.75 / 2.50 s).
+    if (s->cfg.slice_ms <= 0) s->cfg.slice_ms = 30.0;
     int rc;
     if ((rc = s->E.init()) || (rc = s->E.enable_host_pause()) || (rc = s->E.reserve_jobs(s->cfg.max_inflight)) ||
         (rc = s->pool.init(s->cfg.mem_budget_bytes))) {

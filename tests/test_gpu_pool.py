@@ -198,3 +198,23 @@ def test_narrowing_hub_roots_equal_the_unnarrowed_run():
         p.close()
     assert out[True]["narrowed"] > 0 and out[False]["narrowed"] == 0
     same(out[True], out[False])
+
+
+def test_hub_root_alone_repeated_against_the_oracle():
+    # One hub root of the Chung-Lu 75k workload (degeneracy position 16400: a
+    # 267-vertex block in the six-word class, 4.9e8 Branch tasks) alone on the
+    # chip: its class keeps all 2,048 waves, nearly all hungry at first, so its
+    # subtrees move through the donation queue between waves on different
+    # XCDs hundreds of thousands of times.  Four runs, each against the
+    # oracle's result for that root (tests/golden/oracle_cl75k_hubs.json).
+    path = os.path.join(GOLDEN, "oracle_cl75k_hubs.json")
+    with open(path) as f:
+        want = json.load(f)["items"]["16400_16401"]
+    g, k, q, _, tr = kplex.workload_cl75k()
+    p = kplex.Prepared(g, k, q, truss=tr, time_limit_s=LIMIT, roots=(16400, 16401))
+    try:
+        for i in range(4):
+            say(f"cl75k hub root 16400, run {i}")
+            same(p.run(), want)
+    finally:
+        p.close()
