@@ -114,13 +114,14 @@ def cpu_baseline(graph, k, q, truss, target_s, window=None):
                        f"preprocessing included per sample chunk")}
 
 
-def batched_vs_plain(ngraphs, time_limit, chunks=1):
+def batched_vs_plain(ngraphs, time_limit, chunks=1, autotune=True):
     """The many-graph workload (Chung-Lu exponent 2.3, average degree 10, n
     log-uniform in [200, 20000], 1-20 planted 25-50-vertex communities at 0.9,
     k in {2,3,4}, q = 3k+8), first `ngraphs` graphs of the 4,096: every graph
     through the plain entry point one after another (kp_enumerate: preprocess,
     upload, decompose, search per graph), then all of them through one
-    kp_enumerate_batch call.  Results must be identical graph by graph."""
+    kp_enumerate_batch call, with the one-word class's grid autotuned on the
+    batch first (untimed).  Results must be identical graph by graph."""
     import kplex
     specs = kplex.batch_graph_specs()[:ngraphs]
     graphs = gen_graphs(specs)
@@ -135,6 +136,20 @@ def batched_vs_plain(ngraphs, time_limit, chunks=1):
             log(f"[bench] batched: plain {i + 1}/{len(graphs)} graphs in {time.perf_counter() - t0:.1f} s")
     plain_s = time.perf_counter() - t0
     log(f"[bench] batched: plain {plain_s:.2f} s")
+    # Autotuning of the batched launch: the one-word class's grid (blocks per
+    # CU) is picked from a few candidates by one untimed batch run each; the
+    # timed run below uses the fastest.
+    tune, tune_runs = None, {}
+    if autotune and chunks == 1:
+        for name, cand in (("default", None), ("bpc5", {1: {"blocks_per_cu": 5}}), ("bpc7", {1: {"blocks_per_cu": 7}})):
+            kplex.set_tune(cand)
+            t1 = time.perf_counter()
+            kplex.enumerate_batch(graphs, [dict(k=k, q=q, time_limit_s=time_limit) for k, q in params])
+            tune_runs[name] = time.perf_counter() - t1
+        best = min(tune_runs, key=tune_runs.get)
+        tune = {"default": None, "bpc5": {1: {"blocks_per_cu": 5}}, "bpc7": {1: {"blocks_per_cu": 7}}}[best]
+        log(f"[bench] batched: tuning runs {', '.join(f'{a} {b:.2f} s' for a, b in tune_runs.items())}; using {best}")
+    kplex.set_tune(tune)
     t0 = time.perf_counter()
     batch, call_gpu_s = [], 0.0
     step = (len(graphs) + chunks - 1) // chunks
@@ -147,6 +162,7 @@ def batched_vs_plain(ngraphs, time_limit, chunks=1):
             log(f"[bench] batched: graphs [{b},{min(len(graphs), b + step)}) in one batch, "
                 f"{time.perf_counter() - t0:.1f} s so far")
     batch_s = time.perf_counter() - t0
+    kplex.set_tune(None)
     log(f"[bench] batched: {chunks} batch call(s) {batch_s:.2f} s")
     for a, b in zip(plain, batch):
         for x in ("count", "hash", "tasks", "valid_roots"):
@@ -156,7 +172,8 @@ def batched_vs_plain(ngraphs, time_limit, chunks=1):
             "plain_s": plain_s, "batched_s": batch_s, "plain_plexes_per_s": count / plain_s,
             "batched_plexes_per_s": count / batch_s, "batched_vs_plain": plain_s / batch_s,
             "plain_gpu_s": sum(r["decompose_ms"] + r["search_ms"] for r in plain) / 1e3,
-            "batched_gpu_s": call_gpu_s,
+            "batched_gpu_s": call_gpu_s, "autotune_runs_s": tune_runs,
+            "autotune_settings": {str(a): b for a, b in tune.items()} if tune else None,
             "identical": True}
 
 
@@ -478,7 +495,8 @@ def main():
     if not args.no_spill_line:
         out["spill"] = spill_line(args.time_limit)
     if args.batch_graphs > 0:
-        out["batched"] = batched_vs_plain(args.batch_graphs, args.time_limit, args.batch_chunks)
+        out["batched"] = batched_vs_plain(args.batch_graphs, args.time_limit, args.batch_chunks,
+                                          autotune=not args.no_autotune)
     if args.served_seconds > 0:
         out["served"] = served_vs_plain(args.served_seconds, args.served_rate, args.time_limit,
                                         slice_ms=args.served_slice_ms)
