@@ -1001,14 +1001,7 @@ struct Worker {
     // ---- root cache: this lane's rows of the current root ----
     // MayFlush: the root may belong to another job than the accumulators
     // (then they are flushed first); callers that pass false have checked
-    // root table entry of a frame's root: the class table, or (one-word class)
-    // a derived-root buffer
-    __device__ __forceinline__ const RootDesc *desc_of(uint32_t ridx) const
-    {
-        if (W == 1 && (ridx & DERIVED_BIT))
-            return a.ddesc[(ridx >> DERIVED_SHIFT) & (NDERIVED_BUF - 1)] + (ridx & DERIVED_INDEX_MASK);
-        return a.desc + ridx;
-    }
+    __device__ __forceinline__ const RootDesc *desc_of(uint32_t ridx) const { return a.desc + ridx; }
     template <bool MayFlush>
     __device__ void load_root(uint32_t ridx)
     {
@@ -1614,7 +1607,7 @@ struct Worker {
             d.q = uint16_t(q);
             N.desc[idx] = d;
             uint64_t *F = N.frames + idx * uint64_t(FrameLayout<1>::FW);
-            F[0] = make_meta(N.tag | uint32_t(idx), f.type, f.r);
+            F[0] = make_meta(N.tag + uint32_t(idx), f.type, f.r);
             F[1] = P1 & live;
             F[2] = C1 & live;
             F[3] = X1 & live;
@@ -2390,7 +2383,7 @@ uint64_t Engine::device_bytes() const
                            &bdesc, &wr_extra, &list, &list_used, &nar_ctr};
     uint64_t s = 0;
     for (const DevBuf *b : all) s += b->bytes;
-    for (const NarBuf &b : nar) s += b.arena.bytes + b.desc.bytes + b.frames.bytes;
+    for (const NarBuf &b : nar) s += b.arena.bytes + b.frames.bytes;
     for (const DevBuf *b : segs) s += b->bytes;
     for (const auto &c : cls) {
         const DevBuf *cb[] = {&c.q, &c.qseq, &c.sched, &c.stack, &c.subq, &c.desc, &c.todo, &c.seed[0], &c.seed[1], &c.dump};
@@ -2456,6 +2449,7 @@ void Engine::reset_tables()
     }
     segs_used = 0;
     for (auto &b : nar) b.in_use = false;
+    nar_base_set = false;
 }
 
 uint64_t *Engine::arena_alloc(uint64_t words)
@@ -2839,16 +2833,15 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary, bool alo
         sa.nar.arena = B.arena.as<uint64_t>();
         sa.nar.arena_cap = B.arena.bytes / 8;
         sa.nar.ctr = E.nar_ctr.as<unsigned long long>();
-        sa.nar.desc = B.desc.as<RootDesc>();
+        const size_t first = E.nar_base + size_t(E.nar_cur) * E.nar_cap;
+        sa.nar.desc = E.cls[0].desc.as<RootDesc>() + first;
         sa.nar.frames = B.frames.as<uint64_t>();
         sa.nar.cap = E.nar_cap;
-        sa.nar.tag = DERIVED_BIT | (uint32_t(E.nar_cur) << DERIVED_SHIFT);
+        sa.nar.tag = uint32_t(first);
         sa.nar.min_slack = o.narrow_min_slack;
         sa.nar.pause_when_full = o.narrow_pause && o.pausable;
         sa.nar.after_ticks = uint64_t(o.narrow_after_ms * double(E.ticks_per_ms));
     }
-    if (W == 1)
-        for (uint32_t b = 0; b < NDERIVED_BUF; b++) sa.ddesc[b] = E.nar[b].desc.as<RootDesc>();
     HIPCHK(hipMemsetAsync(sa.sched, 0, sizeof(Sched), C.stream));
     HIPCHK(hipMemsetAsync(sa.subq, 0, sizeof(SubQ) * NSUBQ, C.stream));
     hipLaunchKernelGGL(iota_kernel, dim3(uint32_t((nslots + 255) / 256)), dim3(256), 0, C.stream, sa.qseq,
@@ -3159,14 +3152,35 @@ int Engine::launch_epoch(const SearchOpts &o)
             if (!nar_cap) {
                 nar_cap = kNarrowRoots;
                 if (const char *e = std::getenv("KP_NARROW_ROOTS"))
-                    nar_cap = std::min<uint32_t>(DERIVED_INDEX_MASK, uint32_t(std::max(1, std::atoi(e))));
+                    nar_cap = std::min<uint32_t>(1u << 28, uint32_t(std::max(1, std::atoi(e))));
             }
             size_t arena = kNarrowArenaBytes;
             if (const char *e = std::getenv("KP_NARROW_ARENA_MB")) arena = size_t(std::max(1, std::atoi(e))) << 20;
             int rc;
-            if ((rc = nar_ctr.ensure(256)) || (rc = B.desc.ensure(size_t(nar_cap) * sizeof(RootDesc))) ||
-                (rc = B.frames.ensure(size_t(nar_cap) * FrameWords(1) * 8)) || (rc = B.arena.ensure(arena)))
+            if ((rc = nar_ctr.ensure(256)) || (rc = B.frames.ensure(size_t(nar_cap) * FrameWords(1) * 8)) ||
+                (rc = B.arena.ensure(arena)))
                 return rc;
+            // The derived roots' entries go past the one-word class's regular
+            // roots, in its own table; the table is reserved once per run for
+            // every buffer (no regular root is added after staging).
+            ClassState &C0 = cls[0];
+            if (!nar_base_set) {
+                nar_base = C0.hdesc.size();
+                const size_t need = (nar_base + size_t(NDERIVED_BUF) * nar_cap) * sizeof(RootDesc);
+                if (C0.desc.bytes < need) {
+                    if ((rc = C0.desc.ensure(need))) return rc;
+                    C0.desc_uploaded = 0;
+                }
+                nar_base_set = true;
+            } else if (C0.hdesc.size() != nar_base) {
+                set_error("one-word root table grew after derived roots were placed");
+                return KP_EINVAL;
+            }
+            if (C0.desc_uploaded < C0.hdesc.size()) {
+                HIPCHK(hipMemcpyAsync(C0.desc.as<RootDesc>() + C0.desc_uploaded, C0.hdesc.data() + C0.desc_uploaded,
+                                      sizeof(RootDesc) * (C0.hdesc.size() - C0.desc_uploaded), hipMemcpyHostToDevice, stream));
+                C0.desc_uploaded = C0.hdesc.size();
+            }
             if ((rc = fill_words(stream, nar_ctr.p, 2, 0))) return rc;
             nar_cur = pick;
         }

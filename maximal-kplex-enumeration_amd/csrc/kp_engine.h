@@ -159,10 +159,12 @@ struct Engine {
     // stay in use until the one-word class has run an epoch through without a
     // pause and holds no pending frame (then every older buffer is free).
     struct NarBuf {
-        DevBuf arena, desc, frames;
+        DevBuf arena, frames;
         bool in_use = false;
     };
     NarBuf nar[kpd::NDERIVED_BUF];
+    size_t nar_base = 0;               // one-word class table index of buffer 0's first derived root
+    bool nar_base_set = false;         // (set once per run, after staging)
     DevBuf nar_ctr;
     int nar_cur = -1;                  // buffer filled by the running epoch (-1 = narrowing off)
     uint32_t nar_cap = 0;              // derived roots per buffer

@@ -107,21 +107,20 @@ struct BuildArgs {
 // of its subtree) fit one word is re-rooted as a derived root of the one-word
 // class.  The derived subgraph keeps those vertices in block order, so the
 // one-word search walks the same tree.  An epoch fills one derived-root buffer
-// (subgraph arena, root table, one frame per root); the one-word class resumes
-// those frames in the next epoch and reads their roots from that buffer.  A
-// frame names a derived root as DERIVED_BIT | buffer << DERIVED_SHIFT | index.
-constexpr uint32_t DERIVED_BIT = 0x80000000u;
-constexpr uint32_t DERIVED_SHIFT = 28;
+// (subgraph arena, root entries, one frame per root); the one-word class
+// resumes those frames in the next epoch.  The root entries are written into a
+// region of the one-word class's own root table reserved past its regular
+// roots (buffer b from base + b * cap), so its frames name derived roots like
+// any other.
 constexpr uint32_t NDERIVED_BUF = 4;
-constexpr uint32_t DERIVED_INDEX_MASK = (1u << DERIVED_SHIFT) - 1u;
 struct NarrowArgs {
     uint64_t *arena;              // derived subgraphs, bump-allocated
     uint64_t arena_cap;           // words
     unsigned long long *ctr;      // [0] arena words reserved, [1] derived roots reserved
-    RootDesc *desc;               // derived root i
+    RootDesc *desc;               // derived root i: one-word class table entry tag + i
     uint64_t *frames;             // its frame (one-word layout, FrameWords(1) words)
     uint32_t cap;                 // derived roots (0 = narrowing off)
-    uint32_t tag;                 // DERIVED_BIT | buffer << DERIVED_SHIFT
+    uint32_t tag;                 // the buffer's first index in the one-word class table
     uint32_t min_slack;           // only frames with |P| + |C ∪ D| >= q + min_slack are narrowed
     uint32_t pause_when_full;     // a wave that finds the buffer full pauses its class
     uint64_t after_ticks;         // narrowing starts this long after the epoch's first wave
@@ -158,7 +157,6 @@ struct SearchArgs {
     uint32_t retire_margin;    // hungry waves kept beyond the busy ones once nothing is left to start
     uint64_t retire_ticks;     // wall-clock ticks a wave stays hungry before it may retire
     NarrowArgs nar;            // wide classes only
-    const RootDesc *ddesc[NDERIVED_BUF];   // one-word class: derived root tables
 };
 
 struct RootKernArgs {
