@@ -787,12 +787,26 @@ struct Worker {
         if (s >= a.back_cap) s -= a.back_cap;
         return back + size_t(s) * FW;
     }
-    __device__ __forceinline__ uint32_t depth() const { return rn + bn; }
+    // One-word frames: the LDS ring holds a wave's whole stack (launches size
+    // it for the DFS depth bound), so the HBM backing stack is never used.
+    static constexpr bool kRingOnly = W == 1;
+    __device__ __forceinline__ uint32_t depth() const
+    {
+        if constexpr (kRingOnly) return rn;
+        return rn + bn;
+    }
 
     // a free ring slot on top: a full ring moves its oldest frame to the HBM
     // backing stack
     __device__ __forceinline__ bool make_room()
     {
+        if constexpr (kRingOnly) {
+            if (rn == a.ring_cap) {
+                set_error(KP_EOVERFLOW);
+                return false;
+            }
+            return true;
+        }
         if (rn == a.ring_cap) {
             if (bn == a.back_cap) {
                 set_error(KP_EOVERFLOW);
@@ -832,6 +846,11 @@ struct Worker {
     }
     __device__ void pop_top(Frame<W> &f)
     {
+        if constexpr (kRingOnly) {
+            rn--;
+            load_slot(ring_slot(rn), f);
+            return;
+        }
         if (rn) {
             rn--;
             load_slot(ring_slot(rn), f);
@@ -843,6 +862,12 @@ struct Worker {
     // the oldest frame: the largest pending subtree
     __device__ void pop_bottom(Frame<W> &f)
     {
+        if (kRingOnly || bn == 0) {
+            load_slot(ring_slot(0), f);
+            rb = (rb + 1 == a.ring_cap) ? 0 : rb + 1;
+            rn--;
+            return;
+        }
         if (bn > 0) {
             load_slot(back_slot(0), f);
             bb = (bb + 1 == a.back_cap) ? 0 : bb + 1;
@@ -2717,6 +2742,7 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary, bool alo
     // LDS ring: 8 KiB of frames per wave unless tuned; at most 64 KiB per block
     const uint32_t ring_max = std::max<uint32_t>(4, (64 * 1024) / (FW * 8 * 4));
     uint32_t ring = T.ring_frames ? std::max<uint32_t>(4, T.ring_frames) : std::max<uint32_t>(16, 8192 / (FW * 8));
+    if (W == 1) ring = need;   // the one-word kernel keeps its whole stack in the ring
     ring = std::min(std::min(ring, ring_max), need);
     const uint32_t back = need > ring ? need - ring + 1 : 0;
     const size_t dyn = size_t(ring) * FW * 8 * 4;
@@ -3892,6 +3918,7 @@ int dev_autotune(kp_prepared *h, kp_tune *out)
             if ((rc = trial(x))) return rc;
         }
         for (uint32_t rf : {8u, 32u, 64u}) {
+            if (kClassW[c] == 1) break;   // the one-word ring always holds the whole stack
             ClassTune x = best;
             x.ring_frames = rf;
             if ((rc = trial(x))) return rc;
