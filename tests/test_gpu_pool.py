@@ -188,7 +188,8 @@ def test_narrowing_hub_roots_equal_the_unnarrowed_run():
     # Hub roots of the Chung-Lu 75k workload (degeneracy positions [16400,16405):
     # blocks of 250-270 vertices, four- and six-word classes, 3.1e9 Branch tasks).
     # Frames whose live set fits 64 vertices become derived roots of the one-word
-    # class; count, hash and the task count (the tree) must not change.
+    # class; count, hash and the task count (the tree) must not change, and both
+    # runs must equal the oracle (tests/golden/oracle_cl75k_hubs.json).
     g, k, q, _, tr = kplex.workload_cl75k()
     out = {}
     for narrow in (True, False):
@@ -198,6 +199,16 @@ def test_narrowing_hub_roots_equal_the_unnarrowed_run():
         p.close()
     assert out[True]["narrowed"] > 0 and out[False]["narrowed"] == 0
     same(out[True], out[False])
+    # and both against the oracle: [16400,16401) + [16401,16405) from its fixtures
+    with open(os.path.join(GOLDEN, "oracle_cl75k_hubs.json")) as f:
+        items = json.load(f)["items"]
+    parts = [items["16400_16401"], items["16401_16405"]]
+    m = (1 << 64) - 1
+    hb = sum(int(x["hash"][:16], 16) for x in parts) & m
+    ha = sum(int(x["hash"][16:], 16) for x in parts) & m
+    assert out[True]["count"] == sum(x["count"] for x in parts)
+    assert out[True]["tasks"] == sum(x["tasks"] for x in parts)
+    assert out[True]["hash"] == f"{hb:016x}{ha:016x}"
 
 
 def test_hub_root_alone_repeated_against_the_oracle():
