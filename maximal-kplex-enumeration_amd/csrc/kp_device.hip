@@ -521,6 +521,14 @@ struct FrameLayout {
 };
 
 
+// writes the uniform 64-bit value v into lane L of the register pair (lo, hi)
+template <int L>
+__device__ __forceinline__ void wlane(uint32_t &lo, uint32_t &hi, uint64_t v)
+{
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(lo) : "s"(uint32_t(v)), "i"(L));
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(hi) : "s"(uint32_t(v >> 32)), "i"(L));
+}
+
 template <int W>
 __device__ __forceinline__ uint64_t rdl64(uint64_t x, uint32_t l)
 {
@@ -1375,6 +1383,120 @@ struct Worker {
         return true;
     }
 
+    // One-word Branch frames of the current root, back to back.  The same
+    // steps as branch() for W = 1, with the frame in scalar registers:
+    // - the include child's degrees into P2 are dP plus the pivot's bit of
+    //   the lane's row (the pivot is a candidate, not in P), not a new count;
+    // - the exclude child goes straight into the LDS ring slot rn, its words
+    //   (meta, P, C0, X|pivot) written into lanes 0-3 of one register pair
+    //   whose lane 0 (the meta word) is set once (every lane starts with it);
+    // - a popped slot is taken only if it is a Branch frame of this root;
+    //   any other frame (a k-search frame, another root's) stays on the
+    //   stack for the general loop.
+    // Runs until nfr reaches burst (the include child, if any, is left in
+    // f and the return value is true), the stack runs dry, the top frame is
+    // not one this loop takes, or an error is set (false in those cases).
+    __device__ bool fast_run(Frame<W> &f, uint32_t &nfr, uint32_t burst)
+    {
+        static_assert(W == 1, "one-word frames only");
+        uint64_t P = f.P[0], C = f.C[0], X = f.X[0];
+        const uint64_t meta = make_meta(cur, FT_BRANCH, 0);
+        const uint64_t A = adjR[0][0];
+        const uint32_t lofs = min(ln, uint32_t(FW - 1));
+        uint32_t slo = uint32_t(meta), shi = uint32_t(meta >> 32);
+        for (;;) {
+            bool child = false;
+            const uint32_t ps = __popcll(P), cs = __popcll(C);
+            if (int(ps + cs) >= q) {
+                if (cs == 0) {
+                    if (X == 0 && int(ps) >= q && maximal(&P, ps)) record(&P, ps);
+                } else {
+                    const uint64_t PC = P | C;
+                    const uint32_t dG = __popcll(A & PC);
+                    uint32_t key = inmask(P) ? (dG << 10) | ln : ~0u;
+                    key = rfl(wmin32(key));
+                    const int minP = int(key >> 10);
+                    if (minP + k >= max(q, int(ps))) {
+                        const uint32_t dP = __popcll(A & P);
+                        const uint32_t kk = (dG << 20) | (dP << 10) | ln;
+                        uint32_t pivot = ~0u;
+                        if (minP + k < int(ps + cs)) {
+                            const uint64_t adjPP = rdl64<W>(A, key & 1023u);
+                            const uint32_t k2 = rfl(wmin32(inmask(C & ~adjPP) ? kk : ~0u));
+                            if (k2 == ~0u) set_error(KP_EINVAL);   // unreachable while the degrees are consistent
+                            else pivot = k2 & 1023u;
+                        } else {
+                            const uint32_t k2 = rfl(wmin32(inmask(C) ? kk : ~0u));
+                            const int mn = min(minP, int(k2 >> 20));
+                            if (mn >= int(ps + cs) - k) {
+                                // P ∪ C is a k-plex (isKplexPC2 :600-622 over X, then maximality)
+                                const uint32_t tot = ps + cs;
+                                const uint64_t Sat = PC & __ballot(int(dG) + k == int(tot));
+                                const uint64_t ext = X & __ballot(int(dG) + k >= int(tot) + 1 && (Sat & ~A) == 0);
+                                if (ext == 0 && maximal(&PC, tot)) record(&PC, tot);
+                            } else {
+                                pivot = k2 & 1023u;
+                            }
+                        }
+                        if (pivot != ~0u) {
+                            const uint64_t pb = 1ull << pivot;
+                            const uint64_t C0 = C & ~pb;
+                            tk++;
+                            if (rn == a.ring_cap) {
+                                set_error(KP_EOVERFLOW);
+                            } else {
+                                const uint64_t XE = X | pb;
+                                wlane<1>(slo, shi, P);
+                                wlane<2>(slo, shi, C0);
+                                wlane<3>(slo, shi, XE);
+                                ring_slot(rn)[lofs] = (uint64_t(shi) << 32) | slo;
+                                rn++;
+                                // include child (:1273-1465)
+                                const uint64_t P2 = P | pb;
+                                const uint32_t ps2 = ps + 1;
+                                const int dP2 = int(dP + uint32_t((A >> pivot) & 1ull));
+                                const uint64_t Sat = P2 & __ballot(dP2 + k == int(ps2));
+                                const uint64_t OK = __ballot(dP2 + k >= int(ps2) + 1 && (Sat & ~A) == 0);
+                                const uint64_t C2 = C0 & rdl64<W>(c2R[0][0], pivot) & OK;
+                                const uint64_t PC2 = P2 | C2;
+                                if ((P2 & __ballot(int(__popcll(A & PC2)) + k < q)) == 0) {
+                                    X = X & rdl64<W>(c4R[0][0], pivot) & OK;
+                                    P = P2;
+                                    C = C2;
+                                    tk++;
+                                    child = true;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            nfr++;
+            if (err) return false;
+            if (nfr == burst) {
+                if (child) {
+                    f.P[0] = P;
+                    f.C[0] = C;
+                    f.X[0] = X;
+                    f.D[0] = 0;
+                    f.root = cur;
+                    f.type = FT_BRANCH;
+                    f.r = 0;
+                }
+                return child;
+            }
+            if (!child) {
+                if (rn == 0) return false;
+                const uint64_t x = ring_slot(rn - 1)[lofs];
+                if (rdl64<W>(x, 0) != meta) return false;
+                rn--;
+                P = rdl64<W>(x, 1);
+                C = rdl64<W>(x, 2);
+                X = rdl64<W>(x, 3);
+            }
+        }
+    }
+
     // emit a Branch task from the k-search (state-0 emit, :2005-2082) as the next frame
     __device__ __forceinline__ bool emit0(uint32_t root, const uint64_t *P, const uint64_t *C, const uint64_t *X,
                                           Frame<W> &nf)
@@ -1971,6 +2093,16 @@ struct Worker {
                     // keep the copy, which the register allocator handles
                     // better (75k window: 303 vs 309 ms in place)
                     have = (f.type == FT_KSEARCH) ? ksearch(f, f) : branch(f, f);
+                } else if (!LISTS && f.type == FT_BRANCH) {
+                    // counts its own frames (nfr) and pops its own stack
+                    have = fast_run(f, nfr, burst);
+                    if (err || nfr == burst) break;
+                    if (!have) {
+                        if (depth() == 0) break;
+                        pop_top(f);
+                        have = true;
+                    }
+                    continue;
                 } else {
                     have = (f.type == FT_KSEARCH) ? ksearch(f, nf) : branch(f, nf);
                     if (have) f = nf;
