@@ -1387,25 +1387,39 @@ struct Worker {
     // steps as branch() for W = 1, with the frame in scalar registers:
     // - the include child's degrees into P2 are dP plus the pivot's bit of
     //   the lane's row (the pivot is a candidate, not in P), not a new count;
-    // - the exclude child goes straight into the LDS ring slot rn, its words
-    //   (meta, P, C0, X|pivot) written into lanes 0-3 of one register pair
-    //   whose lane 0 (the meta word) is set once (every lane starts with it);
+    // - the include child is built before the exclude child is stored: when
+    //   the include child is pruned, the exclude child is the next frame
+    //   and never goes through the ring (branch() pushes it and pops it);
+    // - a stored exclude child goes straight into the LDS ring slot above
+    //   the top (tix), its words (meta, P, C0, X|pivot) written into lanes
+    //   0-3 of one register pair whose lane 0 (the meta word) is set once
+    //   (every lane starts with it);
+    // - the top slot is read at the start of every frame, so that a frame
+    //   that ends without a child finds the next one already loaded (the
+    //   read is unused when the frame stores a child, or the stack is empty);
     // - a popped slot is taken only if it is a Branch frame of this root;
     //   any other frame (a k-search frame, another root's) stays on the
     //   stack for the general loop.
-    // Runs until nfr reaches burst (the include child, if any, is left in
-    // f and the return value is true), the stack runs dry, the top frame is
-    // not one this loop takes, or an error is set (false in those cases).
+    // Runs until nfr reaches burst (the next frame, if any, is left in f and
+    // the return value is true), the stack runs dry, the top frame is not
+    // one this loop takes, or an error is set (false in those cases).
     __device__ bool fast_run(Frame<W> &f, uint32_t &nfr, uint32_t burst)
     {
         static_assert(W == 1, "one-word frames only");
         uint64_t P = f.P[0], C = f.C[0], X = f.X[0];
         const uint64_t meta = make_meta(cur, FT_BRANCH, 0);
         const uint64_t A = adjR[0][0];
+        const uint32_t cap = a.ring_cap;
         const uint32_t lofs = min(ln, uint32_t(FW - 1));
         uint32_t slo = uint32_t(meta), shi = uint32_t(meta >> 32);
+        // ring index of the top slot (of the slot below the bottom when the
+        // stack is empty)
+        uint32_t tix = rb + rn + cap - 1;
+        if (tix >= cap) tix -= cap;
+        if (tix >= cap) tix -= cap;
         for (;;) {
-            bool child = false;
+            const uint64_t xs = ring[size_t(tix) * FW + lofs];
+            bool next = false;   // (P, C, X) holds the next frame
             const uint32_t ps = __popcll(P), cs = __popcll(C);
             if (int(ps + cs) >= q) {
                 if (cs == 0) {
@@ -1439,33 +1453,38 @@ struct Worker {
                             }
                         }
                         if (pivot != ~0u) {
+                            // branchInCand2 (:1467-1472): exclude child (P, C0, X|pivot),
+                            // include child (:1273-1465)
                             const uint64_t pb = 1ull << pivot;
                             const uint64_t C0 = C & ~pb;
                             tk++;
-                            if (rn == a.ring_cap) {
-                                set_error(KP_EOVERFLOW);
-                            } else {
-                                const uint64_t XE = X | pb;
-                                wlane<1>(slo, shi, P);
-                                wlane<2>(slo, shi, C0);
-                                wlane<3>(slo, shi, XE);
-                                ring_slot(rn)[lofs] = (uint64_t(shi) << 32) | slo;
-                                rn++;
-                                // include child (:1273-1465)
-                                const uint64_t P2 = P | pb;
-                                const uint32_t ps2 = ps + 1;
-                                const int dP2 = int(dP + uint32_t((A >> pivot) & 1ull));
-                                const uint64_t Sat = P2 & __ballot(dP2 + k == int(ps2));
-                                const uint64_t OK = __ballot(dP2 + k >= int(ps2) + 1 && (Sat & ~A) == 0);
-                                const uint64_t C2 = C0 & rdl64<W>(c2R[0][0], pivot) & OK;
-                                const uint64_t PC2 = P2 | C2;
-                                if ((P2 & __ballot(int(__popcll(A & PC2)) + k < q)) == 0) {
+                            const uint64_t P2 = P | pb;
+                            const uint32_t ps2 = ps + 1;
+                            const int dP2 = int(dP + uint32_t((A >> pivot) & 1ull));
+                            const uint64_t Sat = P2 & __ballot(dP2 + k == int(ps2));
+                            const uint64_t OK = __ballot(dP2 + k >= int(ps2) + 1 && (Sat & ~A) == 0);
+                            const uint64_t C2 = C0 & rdl64<W>(c2R[0][0], pivot) & OK;
+                            const uint64_t PC2 = P2 | C2;
+                            const uint64_t XE = X | pb;
+                            next = true;
+                            if ((P2 & __ballot(int(__popcll(A & PC2)) + k < q)) == 0) {
+                                if (rn == cap) {
+                                    set_error(KP_EOVERFLOW);
+                                } else {
+                                    tix = tix + 1 == cap ? 0 : tix + 1;
+                                    wlane<1>(slo, shi, P);
+                                    wlane<2>(slo, shi, C0);
+                                    wlane<3>(slo, shi, XE);
+                                    ring[size_t(tix) * FW + lofs] = (uint64_t(shi) << 32) | slo;
+                                    rn++;
                                     X = X & rdl64<W>(c4R[0][0], pivot) & OK;
                                     P = P2;
                                     C = C2;
                                     tk++;
-                                    child = true;
                                 }
+                            } else {
+                                C = C0;
+                                X = XE;
                             }
                         }
                     }
@@ -1474,7 +1493,7 @@ struct Worker {
             nfr++;
             if (err) return false;
             if (nfr == burst) {
-                if (child) {
+                if (next) {
                     f.P[0] = P;
                     f.C[0] = C;
                     f.X[0] = X;
@@ -1483,16 +1502,16 @@ struct Worker {
                     f.type = FT_BRANCH;
                     f.r = 0;
                 }
-                return child;
+                return next;
             }
-            if (!child) {
+            if (!next) {
                 if (rn == 0) return false;
-                const uint64_t x = ring_slot(rn - 1)[lofs];
-                if (rdl64<W>(x, 0) != meta) return false;
+                if (rdl64<W>(xs, 0) != meta) return false;
                 rn--;
-                P = rdl64<W>(x, 1);
-                C = rdl64<W>(x, 2);
-                X = rdl64<W>(x, 3);
+                tix = tix == 0 ? cap - 1 : tix - 1;
+                P = rdl64<W>(xs, 1);
+                C = rdl64<W>(xs, 2);
+                X = rdl64<W>(xs, 3);
             }
         }
     }
