@@ -630,7 +630,10 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary, bool alo
     sa.qcap = T.queue_slots ? std::max<uint32_t>(16, T.queue_slots) : std::max<uint32_t>(256, sa.nwaves * 4 / NSUBQ);
     sa.ring_cap = ring;
     sa.back_cap = back;
-    sa.donate_period = T.donate_period ? T.donate_period : 256u;
+    // donation check period in frames: the one-word frame loop runs a frame
+    // in a fraction of the wide classes' time (75k window: 179 ms at 512
+    // against 188 ms at 256)
+    sa.donate_period = T.donate_period ? T.donate_period : (W == 1 ? 512u : 256u);
     sa.donate_target = std::max<uint32_t>(16, sa.nwaves / 4);
     sa.retire_margin = std::max<uint32_t>(32, sa.nwaves / 16);
     // a class alone on the chip keeps its waves: retiring them frees slots
@@ -1746,7 +1749,8 @@ int dev_autotune(kp_prepared *h, kp_tune *out)
             x.blocks_per_cu = nb;
             if ((rc = trial(x))) return rc;
         }
-        for (uint32_t dp : {32u, 64u, 128u, 512u, 1024u}) {
+        for (uint32_t dp : {32u, 64u, 128u, 256u, 512u, 1024u}) {
+            if (dp == (kClassW[c] == 1 ? 512u : 256u)) continue;   // the default
             ClassTune x = best;
             x.donate_period = dp;
             if ((rc = trial(x))) return rc;
