@@ -37,6 +37,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <utility>
 
 namespace kpd {
 
@@ -691,6 +692,9 @@ struct Worker {
     // lane's store is harmless.  Wider slots go through lane 0.  Loaded words
     // are broadcast with readlane / readfirstlane.
     static constexpr bool kLaneSlots = FW <= 64 && W <= 4;
+    // Branch frames of the list-free kernels of these classes run in a
+    // dedicated loop (fast_run, fast_run_w)
+    static constexpr bool kFastRun = !LISTS && W <= 2;
     // Shared: a queue slot that another wave reads once its sequence number
     // is published.  Its words are written with agent-scope stores, which on
     // MI355X write through the XCD's L2, so the reader (another XCD) finds
@@ -1516,6 +1520,194 @@ struct Worker {
         }
     }
 
+    // fast_run for frames of two or more words: the same steps as branch(),
+    // with the frame in scalar registers, the read-ahead of the top ring
+    // slot and the exclude child run directly when the include child is
+    // pruned (see fast_run).  The ring is backed by the HBM stack here: a full
+    // ring moves its oldest frame there (make_room, which leaves the top
+    // slot's index unchanged), and when the ring runs dry the general loop
+    // pops from the HBM stack.
+    template <int B, int... I>
+    static __device__ __forceinline__ void wlanes(uint32_t &lo, uint32_t &hi, const uint64_t *v,
+                                                  std::integer_sequence<int, I...>)
+    {
+        (wlane<B + I>(lo, hi, v[I]), ...);
+    }
+    __device__ bool fast_run_w(Frame<W> &f, uint32_t &nfr, uint32_t burst)
+    {
+        static_assert(W >= 2 && kLaneSlots, "lane-slot frames of two or more words");
+        constexpr auto seq = std::make_integer_sequence<int, W>{};
+        uint64_t P[W], C[W], X[W];
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            P[w] = f.P[w];
+            C[w] = f.C[w];
+            X[w] = f.X[w];
+        }
+        const uint64_t meta = make_meta(cur, FT_BRANCH, 0);
+        const uint32_t cap = a.ring_cap;
+        const uint32_t lofs = min(ln, uint32_t(FW - 1));
+        uint32_t slo = uint32_t(meta), shi = uint32_t(meta >> 32);
+        uint32_t tix = rb + rn + cap - 1;
+        if (tix >= cap) tix -= cap;
+        if (tix >= cap) tix -= cap;
+        for (;;) {
+            const uint64_t xs = ring[size_t(tix) * FW + lofs];
+            bool next = false;   // (P, C, X) holds the next frame
+            const uint32_t ps = popc(P), cs = popc(C);
+            if (int(ps + cs) >= q) {
+                if (cs == 0) {
+                    if (popc(X) == 0 && int(ps) >= q && maximal(P, ps)) record(P, ps);
+                } else {
+                    uint64_t PC[W];
+#pragma unroll
+                    for (int w = 0; w < W; w++) PC[w] = P[w] | C[w];
+                    uint32_t dG[W], dP[W];
+#pragma unroll
+                    for (int j = 0; j < W; j++) {
+                        dG[j] = deg_in(j, PC);
+                        dP[j] = deg_in(j, P);
+                    }
+                    uint32_t key = ~0u;
+#pragma unroll
+                    for (int j = 0; j < W; j++)
+                        if (has(P, j)) key = min(key, (dG[j] << 10) | (64u * j + ln));
+                    key = rfl(wmin32(key));
+                    const int minP = int(key >> 10);
+                    if (minP + k >= max(q, int(ps))) {
+                        uint32_t pivot = ~0u;
+                        if (minP + k < int(ps + cs)) {
+                            uint64_t adjPP[W];
+                            adj_row(key & 1023u, adjPP);
+                            uint32_t k2 = ~0u;
+#pragma unroll
+                            for (int j = 0; j < W; j++)
+                                if (inmask(C[j] & ~adjPP[j])) k2 = min(k2, (dG[j] << 20) | (dP[j] << 10) | (64u * j + ln));
+                            k2 = rfl(wmin32(k2));
+                            if (k2 == ~0u) set_error(KP_EINVAL);   // unreachable while the degrees are consistent
+                            else pivot = k2 & 1023u;
+                        } else {
+                            uint32_t k2 = ~0u;
+#pragma unroll
+                            for (int j = 0; j < W; j++)
+                                if (has(C, j)) k2 = min(k2, (dG[j] << 20) | (dP[j] << 10) | (64u * j + ln));
+                            k2 = rfl(wmin32(k2));
+                            const int mn = min(minP, int(k2 >> 20));
+                            if (mn >= int(ps + cs) - k) {
+                                // P ∪ C is a k-plex (isKplexPC2 :600-622 over X, then maximality)
+                                const uint32_t tot = ps + cs;
+                                uint64_t Sat[W];
+#pragma unroll
+                                for (int j = 0; j < W; j++) Sat[j] = PC[j] & __ballot(int(dG[j]) + k == int(tot));
+                                uint64_t ext = 0;
+#pragma unroll
+                                for (int j = 0; j < W; j++)
+                                    ext |= X[j] & __ballot(int(dG[j]) + k >= int(tot) + 1 && subset_of_adj(j, Sat));
+                                if (ext == 0 && maximal(PC, tot)) record(PC, tot);
+                            } else {
+                                pivot = k2 & 1023u;
+                            }
+                        }
+                        if (pivot != ~0u) {
+                            // branchInCand2 (:1467-1472): exclude child (P, C0, X|pivot),
+                            // include child (:1273-1465)
+                            const uint32_t pw = pivot >> 6, pl = pivot & 63u;
+                            const uint64_t pb = 1ull << pl;
+                            uint64_t C0[W], P2[W], XE[W];
+#pragma unroll
+                            for (int w = 0; w < W; w++) {
+                                const uint64_t m = w == int(pw) ? pb : 0ull;
+                                C0[w] = C[w] & ~m;
+                                P2[w] = P[w] | m;
+                                XE[w] = X[w] | m;
+                            }
+                            tk++;
+                            const uint32_t ps2 = ps + 1;
+                            // degrees into P2: the pivot (a candidate) adds its bit of each row
+                            int dP2[W];
+#pragma unroll
+                            for (int j = 0; j < W; j++) {
+                                uint64_t col = 0;
+#pragma unroll
+                                for (int w = 0; w < W; w++)
+                                    if (w == int(pw)) col = adjR[j][w];
+                                dP2[j] = int(dP[j] + uint32_t((col >> pl) & 1ull));
+                            }
+                            uint64_t Sat[W], OK[W];
+#pragma unroll
+                            for (int j = 0; j < W; j++) Sat[j] = P2[j] & __ballot(dP2[j] + k == int(ps2));
+#pragma unroll
+                            for (int j = 0; j < W; j++)
+                                OK[j] = __ballot(dP2[j] + k >= int(ps2) + 1 && subset_of_adj(j, Sat));
+                            uint64_t c2v[W], C2[W], PC2[W];
+                            cls2_row(pivot, c2v);
+#pragma unroll
+                            for (int w = 0; w < W; w++) {
+                                C2[w] = C0[w] & c2v[w] & OK[w];
+                                PC2[w] = P2[w] | C2[w];
+                            }
+                            next = true;
+                            if (ub_ok(P2, PC2)) {
+                                if (make_room()) {
+                                    tix = tix + 1 == cap ? 0 : tix + 1;
+                                    wlanes<1>(slo, shi, P, seq);
+                                    wlanes<1 + W>(slo, shi, C0, seq);
+                                    wlanes<1 + 2 * W>(slo, shi, XE, seq);
+                                    ring[size_t(tix) * FW + lofs] = (uint64_t(shi) << 32) | slo;
+                                    rn++;
+                                    uint64_t c4v[W];
+                                    cls4_row(pivot, c4v);
+#pragma unroll
+                                    for (int w = 0; w < W; w++) {
+                                        X[w] = X[w] & c4v[w] & OK[w];
+                                        P[w] = P2[w];
+                                        C[w] = C2[w];
+                                    }
+                                    tk++;
+                                }
+                            } else {
+#pragma unroll
+                                for (int w = 0; w < W; w++) {
+                                    C[w] = C0[w];
+                                    X[w] = XE[w];
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            nfr++;
+            if (err) return false;
+            if (nfr == burst) {
+                if (next) {
+#pragma unroll
+                    for (int w = 0; w < W; w++) {
+                        f.P[w] = P[w];
+                        f.C[w] = C[w];
+                        f.X[w] = X[w];
+                        f.D[w] = 0;
+                    }
+                    f.root = cur;
+                    f.type = FT_BRANCH;
+                    f.r = 0;
+                }
+                return next;
+            }
+            if (!next) {
+                if (rn == 0) return false;
+                if (rdl64<W>(xs, 0) != meta) return false;
+                rn--;
+                tix = tix == 0 ? cap - 1 : tix - 1;
+#pragma unroll
+                for (int w = 0; w < W; w++) {
+                    P[w] = rdl64<W>(xs, 1 + w);
+                    C[w] = rdl64<W>(xs, 1 + W + w);
+                    X[w] = rdl64<W>(xs, 1 + 2 * W + w);
+                }
+            }
+        }
+    }
+
     // emit a Branch task from the k-search (state-0 emit, :2005-2082) as the next frame
     __device__ __forceinline__ bool emit0(uint32_t root, const uint64_t *P, const uint64_t *C, const uint64_t *X,
                                           Frame<W> &nf)
@@ -2106,22 +2298,25 @@ struct Worker {
                 if constexpr (W >= 2) narrowed = nar_on == 1 && try_narrow(f);
                 if (narrowed) {
                     have = false;
+                } else if (kFastRun && f.type == FT_BRANCH && nar_on != 1) {
+                    if constexpr (kFastRun) {
+                        // counts its own frames (nfr) and pops its own stack
+                        if constexpr (W == 1) have = fast_run(f, nfr, burst);
+                        else have = fast_run_w(f, nfr, burst);
+                        if (err || nfr == burst) break;
+                        if (!have) {
+                            if (depth() == 0) break;
+                            pop_top(f);
+                            have = true;
+                        }
+                    }
+                    continue;
                 } else if constexpr (W >= 2) {
                     // wide frames: the child is built in place (soc-epinions,
                     // mostly two-word frames: 188 -> 160 ms); one-word frames
                     // keep the copy, which the register allocator handles
                     // better (75k window: 303 vs 309 ms in place)
                     have = (f.type == FT_KSEARCH) ? ksearch(f, f) : branch(f, f);
-                } else if (!LISTS && f.type == FT_BRANCH) {
-                    // counts its own frames (nfr) and pops its own stack
-                    have = fast_run(f, nfr, burst);
-                    if (err || nfr == burst) break;
-                    if (!have) {
-                        if (depth() == 0) break;
-                        pop_top(f);
-                        have = true;
-                    }
-                    continue;
                 } else {
                     have = (f.type == FT_KSEARCH) ? ksearch(f, nf) : branch(f, nf);
                     if (have) f = nf;
