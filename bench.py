@@ -137,17 +137,21 @@ def batched_vs_plain(ngraphs, time_limit, chunks=1, autotune=True):
     plain_s = time.perf_counter() - t0
     log(f"[bench] batched: plain {plain_s:.2f} s")
     # Autotuning of the batched launch: the one-word class's grid (blocks per
-    # CU) is picked from a few candidates by one untimed batch run each; the
-    # timed run below uses the fastest.
+    # CU) and donation period are picked from a few candidates by one untimed
+    # batch run each; the timed run below uses the fastest.
     tune, tune_runs = None, {}
     if autotune and chunks == 1:
-        for name, cand in (("default", None), ("bpc5", {1: {"blocks_per_cu": 5}}), ("bpc7", {1: {"blocks_per_cu": 7}})):
+        cands = {"default": None, "bpc5": {1: {"blocks_per_cu": 5}}, "bpc7": {1: {"blocks_per_cu": 7}},
+                 "dp2048": {1: {"donate_period": 2048}},
+                 "bpc6_dp2048": {1: {"blocks_per_cu": 6, "donate_period": 2048}},
+                 "bpc7_dp2048": {1: {"blocks_per_cu": 7, "donate_period": 2048}}}
+        for name, cand in cands.items():
             kplex.set_tune(cand)
             t1 = time.perf_counter()
             kplex.enumerate_batch(graphs, [dict(k=k, q=q, time_limit_s=time_limit) for k, q in params])
             tune_runs[name] = time.perf_counter() - t1
         best = min(tune_runs, key=tune_runs.get)
-        tune = {"default": None, "bpc5": {1: {"blocks_per_cu": 5}}, "bpc7": {1: {"blocks_per_cu": 7}}}[best]
+        tune = cands[best]
         log(f"[bench] batched: tuning runs {', '.join(f'{a} {b:.2f} s' for a, b in tune_runs.items())}; using {best}")
     kplex.set_tune(tune)
     t0 = time.perf_counter()
