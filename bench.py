@@ -114,7 +114,7 @@ def cpu_baseline(graph, k, q, truss, target_s, window=None):
                        f"preprocessing included per sample chunk")}
 
 
-def batched_vs_plain(ngraphs, time_limit, chunks=1, autotune=True):
+def batched_vs_plain(ngraphs, time_limit, chunks=1, autotune=True, fixed_tune=None):
     """The many-graph workload (Chung-Lu exponent 2.3, average degree 10, n
     log-uniform in [200, 20000], 1-20 planted 25-50-vertex communities at 0.9,
     k in {2,3,4}, q = 3k+8), first `ngraphs` graphs of the 4,096: every graph
@@ -139,8 +139,8 @@ def batched_vs_plain(ngraphs, time_limit, chunks=1, autotune=True):
     # Autotuning of the batched launch: the one-word class's grid (blocks per
     # CU) and donation period are picked from a few candidates by one untimed
     # batch run each; the timed run below uses the fastest.
-    tune, tune_runs = None, {}
-    if autotune and chunks == 1:
+    tune, tune_runs = fixed_tune, {}
+    if autotune and chunks == 1 and fixed_tune is None:
         cands = {"default": None, "bpc5": {1: {"blocks_per_cu": 5}}, "bpc7": {1: {"blocks_per_cu": 7}},
                  "dp2048": {1: {"donate_period": 2048}},
                  "bpc6_dp2048": {1: {"blocks_per_cu": 6, "donate_period": 2048}},
@@ -391,6 +391,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=48,
                     help="graphs of the many-graph workload for the batched-vs-plain line (0 = skip)")
+    ap.add_argument("--batch-tune", default=None, metavar="JSON",
+                    help='launch settings for the batch calls instead of tuning runs, e.g. {"1": {"blocks_per_cu": 7}}')
     ap.add_argument("--batch-chunks", type=int, default=1,
                     help="split the batch into this many consecutive kp_enumerate_batch calls (long runs print per call)")
     ap.add_argument("--served-seconds", type=float, default=1.5,
@@ -499,8 +501,11 @@ def main():
     if not args.no_spill_line:
         out["spill"] = spill_line(args.time_limit)
     if args.batch_graphs > 0:
+        fixed = None
+        if args.batch_tune:
+            fixed = {int(a): b for a, b in json.loads(args.batch_tune).items()}
         out["batched"] = batched_vs_plain(args.batch_graphs, args.time_limit, args.batch_chunks,
-                                          autotune=not args.no_autotune)
+                                          autotune=not args.no_autotune, fixed_tune=fixed)
     if args.served_seconds > 0:
         out["served"] = served_vs_plain(args.served_seconds, args.served_rate, args.time_limit,
                                         slice_ms=args.served_slice_ms)
