@@ -36,6 +36,8 @@ WORKLOADS = {
     "er2000": ("G(2000,0.01) + 20 planted groups of 40, k=3 q=20", "workload_er2000", None),
     "cl320k": ("chung-lu n=320000 gamma=2.0 ~1.05M edges + 1000 communities 30-80 @0.85, k=5 q=30, truss; "
                "roots at degeneracy positions [0,800) of 50565 (1.96e9 tasks)", "workload_cl320k", (0, 800)),
+    "cl320k-full": ("chung-lu n=320000 gamma=2.0 ~1.05M edges + 1000 communities 30-80 @0.85, k=5 q=30, truss",
+                    "workload_cl320k", None),
     "soc-epinions": ("reference dataset soc-epinions (README.md:142), k=4 q=30", None, None),
     "email-euall": ("reference dataset email-euall (README.md:144), k=3 q=12", None, None),
 }
@@ -323,28 +325,41 @@ def small_line(reps=7):
             "gpu_vs_cpu": c_med / g_med, "identical": True}
 
 
-def full_graph_line(time_limit, lo=0, hi=None):
-    """The Chung-Lu 75k workload in full: every root position in [lo, hi)
-    (default: all), enumerated as consecutive windows (windows partition the
-    result, so counts, task counts and hashes add up), each window one kp_run
-    that prints its line when it ends; the last, heaviest positions go in
-    narrower windows.  Over [0, end) the sum is the whole graph's exact count."""
+def full_graph_line(time_limit, lo=0, hi=None, workload="cl75k-full"):
+    """A workload in full (the Chung-Lu 75k graph, or the Chung-Lu 320k graph
+    with truss): every root position in [lo, hi) (default: all), enumerated
+    as consecutive windows (windows partition the result, so counts, task
+    counts and hashes add up), each window one kp_run that prints its line
+    when it ends; the last, heaviest positions go in narrower windows.  Over
+    [0, end) the sum is the whole graph's exact count.  A window that fails
+    (its time limit) ends the line, which then reports the positions done."""
     import kplex
-    desc, fn, _ = WORKLOADS["cl75k-full"]
+    desc, fn, _ = WORKLOADS[workload]
     g, k, q, thres, tr = getattr(kplex, fn)()
     end = len(kplex.preprocess(g, k, q, tr)[0]) - q + 2
     hi = end if hi is None else min(hi, end)
-    cuts = ([0, 9000] + list(range(9500, min(end, 16000), 500)) + list(range(16000, min(end, 16400), 100)) +
-            list(range(16400, end, 25)) + [end, lo, hi])
+    if workload == "cl75k-full":
+        cuts = ([0, 9000] + list(range(9500, min(end, 16000), 500)) + list(range(16000, min(end, 16400), 100)) +
+                list(range(16400, end, 25)))
+    else:
+        cuts = ([0, 800] + list(range(2000, end - 2000, 2000)) + list(range(end - 2000, end - 250, 250)) +
+                list(range(end - 250, end, 25)))
+    cuts += [end, lo, hi]
     cuts = sorted(set(c for c in cuts if lo <= c <= hi))
     windows = list(zip(cuts[:-1], cuts[1:]))
     tot = {"count": 0, "tasks": 0, "hash_a": 0, "hash_b": 0, "search_ms": 0.0, "valid_roots": 0}
     per = []
     t0 = time.perf_counter()
     p = kplex.Prepared(g, k, q, thres, tr, time_limit_s=time_limit)
+    failed = None
     for w in windows:
         p.set_roots(w)
-        r = p.run()
+        try:
+            r = p.run()
+        except kplex.KplexError as e:
+            failed = {"window": list(w), "error": str(e)}
+            log(f"[bench] full graph window {w}: {e}; stopping")
+            break
         for x in ("count", "tasks", "valid_roots"):
             tot[x] += r[x]
         tot["hash_a"] = (tot["hash_a"] + r["hash_a"]) % (1 << 64)
@@ -356,7 +371,10 @@ def full_graph_line(time_limit, lo=0, hi=None):
             f"(total {tot['count']} after {time.perf_counter() - t0:.1f} s)")
     wall = time.perf_counter() - t0
     p.close()
-    return {"workload": desc, "roots": end, "positions": [lo, hi], "windows": len(windows), "count": tot["count"],
+    if failed:
+        hi = failed["window"][0]
+    return {"workload": desc, "roots": end, "positions": [lo, hi], "windows": len(per), "failed": failed,
+            "count": tot["count"],
             "hash": f"{tot['hash_b']:016x}{tot['hash_a']:016x}", "tasks": tot["tasks"],
             "valid_roots": tot["valid_roots"], "gpu_s": tot["search_ms"] / 1e3, "wall_s": wall,
             "plexes_per_s": tot["count"] / wall, "per_window": per}
@@ -411,6 +429,7 @@ def main():
     ap.add_argument("--full-graph", action="store_true",
                     help="also enumerate the whole Chung-Lu 75k graph (every root; many minutes of GPU time)")
     ap.add_argument("--full-graph-only", action="store_true", help="only the whole-graph line, printed as JSON")
+    ap.add_argument("--full-graph-workload", default="cl75k-full", choices=["cl75k-full", "cl320k-full"])
     ap.add_argument("--full-graph-range", default=None, metavar="LO:HI",
                     help="root positions of the whole-graph line (default: all)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -421,7 +440,7 @@ def main():
         return
     fg_range = tuple(int(x) for x in args.full_graph_range.split(":")) if args.full_graph_range else (0, None)
     if args.full_graph_only:
-        print(json.dumps({"full_graph": full_graph_line(args.time_limit, *fg_range)}))
+        print(json.dumps({"full_graph": full_graph_line(args.time_limit, *fg_range, workload=args.full_graph_workload)}))
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -515,7 +534,7 @@ def main():
         out["served"] = served_vs_plain(args.served_seconds, args.served_rate, args.time_limit,
                                         slice_ms=args.served_slice_ms, plain=not args.served_no_plain)
     if args.full_graph:
-        out["full_graph"] = full_graph_line(args.time_limit, *fg_range)
+        out["full_graph"] = full_graph_line(args.time_limit, *fg_range, workload=args.full_graph_workload)
     # the other parts of the metric, at the top level
     out["wall_s_at_exact_count"] = out["wall_s_per_step"]
     out["batched_vs_plain"] = out["batched"]["batched_vs_plain"] if "batched" in out else None
