@@ -424,6 +424,9 @@ def main():
     ap.add_argument("--served-slice-ms", type=float, default=30.0,
                     help="age at which a running epoch is paused to admit waiting queries")
     ap.add_argument("--no-autotune", action="store_true", help="time with the built-in launch settings")
+    ap.add_argument("--tune", default=None, metavar="JSON",
+                    help='fixed launch settings for the headline instead of autotuning, e.g. '
+                         '{"1": {"blocks_per_cu": 6, "donate_period": 2048}}')
     ap.add_argument("--no-spill-line", action="store_true", help="skip the pause/spill line on the 320k workload")
     ap.add_argument("--no-small-line", action="store_true", help="skip the GPU-vs-CPU line on the small planted workload")
     ap.add_argument("--full-graph", action="store_true",
@@ -456,7 +459,11 @@ def main():
     prep = kplex.Prepared(graph, k, q, thres, truss, time_limit_s=args.time_limit, roots=window)
     prep_s = time.perf_counter() - t0
     tune, tune_s, untuned_ms = None, 0.0, None
-    if not args.no_autotune:
+    if args.tune:
+        tune = {int(a): b for a, b in json.loads(args.tune).items()}
+        kplex.set_tune(tune)
+        log(f"[bench] fixed launch settings {tune}")
+    elif not args.no_autotune:
         r = prep.run()
         r = prep.run()
         untuned_ms = r["search_ms"]
