@@ -2419,10 +2419,12 @@ struct Worker {
     }
 };
 
-// Waves per SIMD (list-free kernels, compiler report): one word 7 (68
-// VGPRs; asked for at least 4), two to four words 4 (104-128 VGPRs), six
-// words 2 (189), eight words 2 when asked for (235, no scratch; 1 otherwise,
-// with AGPR spills); sixteen words 1 with 724 bytes of scratch per lane.
+// Waves per SIMD (list-free kernels, compiler report): one word 6-7 (74
+// VGPRs, 71 in the kp_search1.hip build; asked for at least 4), two to four
+// words 4-5 (89-109 VGPRs), six words 2 (170), eight words 2 when asked for
+// (228, no scratch); sixteen words 1 with 4,800 bytes of scratch per lane
+// (628 without the narrowing code, whose gathers need the registers that the
+// sixteen-word frame already fills).
 // V tags a second build of the same kernel (kp_search1.hip: V = 1).
 template <int W, int LW, bool LISTS, int V = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1 ? 4 : (W == 8 ? 2 : 1)))) void search_kernel(SearchArgs a)
