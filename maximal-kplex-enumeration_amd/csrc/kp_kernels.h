@@ -1416,6 +1416,9 @@ struct Worker {
         const uint32_t cap = a.ring_cap;
         const uint32_t lofs = min(ln, uint32_t(FW - 1));
         uint32_t slo = uint32_t(meta), shi = uint32_t(meta >> 32);
+        // slots above the entry depth were written by this loop: Branch frames
+        // of this root, whose meta word needs no check
+        const uint32_t rn0 = rn;
         // ring index of the top slot (of the slot below the bottom when the
         // stack is empty)
         uint32_t tix = rb + rn + cap - 1;
@@ -1466,7 +1469,8 @@ struct Worker {
                             const uint32_t ps2 = ps + 1;
                             const int dP2 = int(dP + uint32_t((A >> pivot) & 1ull));
                             const uint64_t Sat = P2 & __ballot(dP2 + k == int(ps2));
-                            const uint64_t OK = __ballot(dP2 + k >= int(ps2) + 1 && (Sat & ~A) == 0);
+                            uint64_t OK = __ballot(dP2 + k >= int(ps2) + 1);
+                            if (Sat) OK &= __ballot((Sat & ~A) == 0);   // no saturated vertex: nothing to be adjacent to
                             const uint64_t C2 = C0 & rdl64<W>(c2R[0][0], pivot) & OK;
                             const uint64_t PC2 = P2 | C2;
                             const uint64_t XE = X | pb;
@@ -1510,7 +1514,7 @@ struct Worker {
             }
             if (!next) {
                 if (rn == 0) return false;
-                if (rdl64<W>(xs, 0) != meta) return false;
+                if (rn <= rn0 && rdl64<W>(xs, 0) != meta) return false;
                 rn--;
                 tix = tix == 0 ? cap - 1 : tix - 1;
                 P = rdl64<W>(xs, 1);
