@@ -579,6 +579,10 @@ struct Worker {
     uint64_t adjR[W][W], c2R[RW][RW], c4R[RW][RW];
     uint64_t lftR[LW > 0 ? LW : 1][W];
     uint32_t origR[RW];
+    // the two per-vertex hash terms of the rows this lane owns, computed once
+    // per root (list-free kernels with rows in registers)
+    static constexpr bool kHashRegs = kRowRegs && !LISTS;
+    uint64_t hAR[kHashRegs ? RW : 1], hBR[kHashRegs ? RW : 1];
     const uint64_t *Ab = nullptr, *C2b = nullptr, *C4b = nullptr;   // the root's arena rows (uniform)
     const uint32_t *OG = nullptr;
     // Branch tasks since the last fold into Cold::tasks (a 32-bit uniform
@@ -1092,6 +1096,10 @@ struct Worker {
                 }
                 const uint32_t o = og[vv];
                 origR[j] = in ? o : 0u;
+                if constexpr (kHashRegs) {
+                    hAR[j] = mix64(uint64_t(origR[j]) ^ kp::HASH_SA);
+                    hBR[j] = mix64(uint64_t(origR[j]) ^ kp::HASH_SB);
+                }
             }
         }
 #pragma unroll
@@ -1249,9 +1257,14 @@ struct Worker {
 #pragma unroll
         for (int j = 0; j < W; j++)
             if (has(Sx, j)) {
-                const uint64_t o = orig_of(j);
-                sa += mix64(o ^ kp::HASH_SA);
-                sb += mix64(o ^ kp::HASH_SB);
+                if constexpr (kHashRegs) {
+                    sa += hAR[j];
+                    sb += hBR[j];
+                } else {
+                    const uint64_t o = orig_of(j);
+                    sa += mix64(o ^ kp::HASH_SA);
+                    sb += mix64(o ^ kp::HASH_SB);
+                }
             }
         sa = rfl64(wsum64(sa));
         sb = rfl64(wsum64(sb));
