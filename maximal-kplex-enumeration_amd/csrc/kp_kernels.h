@@ -544,7 +544,7 @@ __device__ __forceinline__ uint64_t rdl64(uint64_t x, uint32_t l)
 struct Cold {
     uint64_t t_start, t_mark, busy_t, idle_t;
     uint64_t alg, task0, bpt, frames, pops;
-    uint64_t plexes, ha, hb, tasks;   // accumulators of the current job (see Worker::tk)
+    uint64_t plexes, ha, hb, tasks;   // accumulators of the current job (see Worker::tk, Worker::pacc)
     uint64_t la;                      // the current root's left rows
     uint32_t spills;
     uint64_t hungry_since;   // wall-clock tick at which the wave last became hungry
@@ -583,6 +583,12 @@ struct Worker {
     // per root (list-free kernels with rows in registers)
     static constexpr bool kHashRegs = kRowRegs && !LISTS;
     uint64_t hAR[kHashRegs ? RW : 1], hBR[kHashRegs ? RW : 1];
+    // One-word kernels keep the job's plex count, hash_a and hash_b since
+    // the last flush in lanes 0, 1 and 2 of a register (other lanes hold
+    // nothing of use); wider ones in the LDS record, which leaves their
+    // registers to the frame.
+    static constexpr bool kAccRegs = W == 1;
+    uint64_t pacc = 0;
     const uint64_t *Ab = nullptr, *C2b = nullptr, *C4b = nullptr;   // the root's arena rows (uniform)
     const uint32_t *OG = nullptr;
     // Branch tasks since the last fold into Cold::tasks (a 32-bit uniform
@@ -1024,7 +1030,16 @@ struct Worker {
         fold();
         const uint32_t job = rfl(cold->cjob);
         const uint64_t alg = rfl64(cold->alg);
-        const uint64_t cnt = rfl64(cold->plexes), ha = rfl64(cold->ha), hb = rfl64(cold->hb);
+        uint64_t cnt, ha, hb;
+        if constexpr (kAccRegs) {
+            cnt = rdl64<W>(pacc, 0);
+            ha = rdl64<W>(pacc, 1);
+            hb = rdl64<W>(pacc, 2);
+        } else {
+            cnt = rfl64(cold->plexes);
+            ha = rfl64(cold->ha);
+            hb = rfl64(cold->hb);
+        }
         const uint64_t tasks = rfl64(cold->tasks);
         if (job != 0xffffffffu && (cnt | tasks) && lane0()) {
             JobAcc *J = a.acc + job;
@@ -1034,6 +1049,7 @@ struct Worker {
             atomicAdd(&J->tasks, (unsigned long long)tasks);
             atomicAdd(&J->alg, (unsigned long long)alg);
         }
+        pacc = 0;
         cold->plexes = cold->ha = cold->hb = cold->tasks = 0;
         cold->alg = 0;
         cold->task0 = 0;
@@ -1268,9 +1284,16 @@ struct Worker {
             }
         sa = rfl64(wsum64(sa));
         sb = rfl64(wsum64(sb));
-        cold->plexes = rfl64(cold->plexes) + 1;
-        cold->ha = rfl64(cold->ha) + mix64(sa + uint64_t(s) * kp::HASH_LA);
-        cold->hb = rfl64(cold->hb) + mix64(sb + uint64_t(s) * kp::HASH_LB);
+        if constexpr (kAccRegs) {
+            uint32_t lo = 1u, hi = 0u;   // lane 0: one more plex
+            wlane<1>(lo, hi, rfl64(mix64(sa + uint64_t(s) * kp::HASH_LA)));
+            wlane<2>(lo, hi, rfl64(mix64(sb + uint64_t(s) * kp::HASH_LB)));
+            pacc += (uint64_t(hi) << 32) | lo;
+        } else {
+            cold->plexes = rfl64(cold->plexes) + 1;
+            cold->ha = rfl64(cold->ha) + mix64(sa + uint64_t(s) * kp::HASH_LA);
+            cold->hb = rfl64(cold->hb) + mix64(sb + uint64_t(s) * kp::HASH_LB);
+        }
         if constexpr (LISTS) {
             uint64_t base = 0;
             if (lane0()) base = atomicAdd((unsigned long long *)a.list_used, (unsigned long long)(s + 2));
@@ -2066,6 +2089,7 @@ struct Worker {
     __device__ void discard()
     {
         tk = 0;
+        pacc = 0;
         cold->plexes = cold->ha = cold->hb = cold->tasks = 0;
         cold->alg = 0;
         cold->task0 = 0;
