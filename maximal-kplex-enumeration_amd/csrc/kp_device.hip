@@ -559,6 +559,12 @@ int Engine::stage(const std::vector<JobIn> &jv, std::vector<JobStats> &st,
 
 namespace {
 
+// Untuned, a one-word class with at least this many pending roots (and
+// frames) but fewer than a full grid has waves runs six blocks per CU rather
+// than four: enough roots that the extra waves find work (75k window, 2,094
+// roots: 179 -> 163-167 ms with the longer donation period below).
+constexpr uint64_t kManyRoots = 1024;
+
 template <int W, int LW>
 int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary, bool alone)
 {
@@ -620,7 +626,8 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary, bool alo
         C.demoted = nb < C.nb_max;
     } else {
         C.demoted = false;
-        if (pending < uint64_t(nb) * uint64_t(E.cus) * 4u) nb = std::min(nb, W >= 2 ? 2 : 4);
+        if (pending < uint64_t(nb) * uint64_t(E.cus) * 4u)
+            nb = std::min(nb, W >= 2 ? 2 : (pending >= kManyRoots ? 6 : 4));
     }
     const uint32_t nwg = uint32_t(nb) * uint32_t(E.cus);
     SearchArgs sa{};
@@ -632,8 +639,10 @@ int launch_class(Engine &E, int c, const SearchOpts &o, bool secondary, bool alo
     sa.back_cap = back;
     // donation check period in frames: the one-word frame loop runs a frame
     // in a fraction of the wide classes' time (75k window: 179 ms at 512
-    // against 188 ms at 256)
-    sa.donate_period = T.donate_period ? T.donate_period : (W == 1 ? 512u : 256u);
+    // against 188 ms at 256), and a one-word grid of six or more blocks per
+    // CU, with more waves to feed, checks less often still (75k window at
+    // six blocks: 164-167 ms at 2,048 frames)
+    sa.donate_period = T.donate_period ? T.donate_period : (W == 1 ? (nb >= 6 ? 2048u : 512u) : 256u);
     sa.donate_target = std::max<uint32_t>(16, sa.nwaves / 4);
     sa.retire_margin = std::max<uint32_t>(32, sa.nwaves / 16);
     // a class alone on the chip keeps its waves: retiring them frees slots
